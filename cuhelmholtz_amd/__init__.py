@@ -1,0 +1,131 @@
+"""cuhelmholtz_amd -- MI355X (gfx950) 3D mixed DFT/DCT/DST transforms and
+FFT-diagonalised Helmholtz solves.
+
+Thin Python mirror of the C ABI (include/cuhelmholtz.h) over ctypes.  The
+arrays are torch tensors on the GPU (torch is only the device-memory and
+stream plumbing); every call goes to the HIP library built in-tree
+(libcuhelmholtz_amd.so).  There is no CPU fallback: if the library is missing
+the import of a compute entry point raises.
+
+Reference API mirrored (names, argument meaning, conventions):
+  do_transform(d_data, d_out, NX, NY, NZ, X, Y, Z)  cudasymmfft/funcinterface.h:5-12
+  transform_kind ordinals                            cudasymmfft/global.h:6-15
+  cuda_helmholtz_solver(...)                         cudahelmholtz/cuda_helmholtz_solver.h:5-11
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcuhelmholtz_amd.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+DCT_1, DCT_2, DCT_3, DST_1, DST_2, DST_3, DFT_R2C, DFT_C2R = range(8)
+KIND_NAMES = ["DCT_1", "DCT_2", "DCT_3", "DST_1", "DST_2", "DST_3", "DFT_R2C", "DFT_C2R"]
+
+# bc code -> (forward kind, inverse kind), cudahelmholtz/cuda_helmholtz_solver.cu:78-172
+BC_KINDS = {0: (DFT_R2C, DFT_C2R), 1: (DST_1, DST_1), 2: (DST_3, DST_2), 3: (DCT_1, DCT_1), 4: (DCT_3, DCT_2)}
+
+
+class CHZError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CHZError(f"{LIB_PATH} is missing: run `python -m cuhelmholtz_amd.build` (needs hipcc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, fp, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_float, ctypes.c_int
+        L.chz_do_transform.argtypes = [vp, vp, i, i, i, i, i, i, vp]
+        L.chz_do_transform_f32.argtypes = [vp, vp, i, i, i, i, i, i, vp]
+        L.chz_last_error.restype = ctypes.c_char_p
+        L.chz_axis_config.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                                      ctypes.POINTER(ctypes.c_ulonglong)]
+        _lib = L
+        _declare_solver(L)
+    return _lib
+
+
+def _declare_solver(L):
+    vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    if hasattr(L, "chz_helmholtz_solve"):
+        L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
+    if hasattr(L, "chz_helmholtz_solve_mixed"):
+        L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, d, vp, vp, vp]
+
+
+def header_symbols():
+    """Function names declared in include/*.h with C linkage (chz_*)."""
+    import re
+    names = []
+    with open(os.path.join(INCLUDE, "cuhelmholtz.h")) as f:
+        for m in re.finditer(r"\b(chz_[a-z0-9_]+)\s*\(", f.read()):
+            if m.group(1) not in names:
+                names.append(m.group(1))
+    return names
+
+
+def _check(rc):
+    if rc != 0:
+        raise CHZError(f"status {rc}: {lib().chz_last_error().decode()}")
+
+
+def _stream(t):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _vol_dims(x):
+    if x.dim() != 3:
+        raise ValueError("expected a padded volume shaped (NZ+2, NY+2, NX+2)")
+    NZ, NY, NX = (int(s) - 2 for s in x.shape)
+    return NX, NY, NZ
+
+
+def do_transform(d_data, d_out=None, kinds=(DCT_2, DCT_2, DCT_2)):
+    """3D transform of a padded CUDA tensor shaped (NZ+2, NY+2, NX+2), x fastest.
+
+    d_out None (or d_data): in place.  Otherwise the result is written into
+    d_out.  float64 -> FP64 path, float32 -> FP32 path.  Returns the result tensor.
+    """
+    import torch
+    if not d_data.is_cuda or not d_data.is_contiguous():
+        raise ValueError("d_data must be a contiguous CUDA tensor")
+    NX, NY, NZ = _vol_dims(d_data)
+    out = d_data if d_out is None else d_out
+    if out.shape != d_data.shape or out.dtype != d_data.dtype or not out.is_contiguous():
+        raise ValueError("d_out must match d_data")
+    fn = {torch.float64: lib().chz_do_transform, torch.float32: lib().chz_do_transform_f32}.get(d_data.dtype)
+    if fn is None:
+        raise ValueError("float64 or float32 only")
+    _check(fn(ctypes.c_void_p(d_data.data_ptr()), ctypes.c_void_p(out.data_ptr()), NX, NY, NZ,
+              int(kinds[0]), int(kinds[1]), int(kinds[2]), _stream(d_data)))
+    return out
+
+
+def axis_config(N, prec=64, rows=True):
+    e, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = ctypes.c_ulonglong()
+    _check(lib().chz_axis_config(N, prec, 1 if rows else 0, ctypes.byref(e), ctypes.byref(t), ctypes.byref(l),
+                                 ctypes.byref(p)))
+    radices = []
+    v = p.value
+    while v:
+        radices.append(v & 255)
+        v >>= 8
+    return {"ept": e.value, "tpl": t.value, "lines": l.value, "radices": radices}
+
+
+def in_window(kind, N):
+    return {DST_1: (1, N - 1), DCT_1: (0, N), DCT_2: (0, N - 1), DST_2: (1, N), DCT_3: (0, N - 1),
+            DST_3: (1, N), DFT_R2C: (0, N - 1), DFT_C2R: (0, N + 1)}[kind]
+
+
+def out_window(kind, N):
+    return {DST_1: (1, N - 1), DCT_1: (0, N), DCT_2: (0, N - 1), DST_2: (1, N), DCT_3: (0, N - 1),
+            DST_3: (1, N), DFT_R2C: (0, N + 1), DFT_C2R: (0, N - 1)}[kind]

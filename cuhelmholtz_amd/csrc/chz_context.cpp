@@ -1,0 +1,165 @@
+// chz_context.cpp -- twiddle tables, axis configurations, workspaces, errors.
+#include "chz_context.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <tuple>
+
+namespace chz {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+const char *last_error() { return g_err.c_str(); }
+
+bool radix_plan(int M, int ept, unsigned long long *plan)
+{
+    static const int radices[] = {16, 12, 8, 6, 4, 3, 2};
+    unsigned long long p = 0;
+    int rem = M, st = 0;
+    while (rem > 1) {
+        int pick = 0;
+        for (int R : radices)
+            if (ept % R == 0 && rem % R == 0) { pick = R; break; }
+        if (!pick || st >= 8) return false;
+        p |= (unsigned long long)pick << (8 * st++);
+        rem /= pick;
+    }
+    *plan = p;
+    return true;
+}
+
+Context::Context(int dev) : dev_(dev)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        lds_cap_ = (size_t)v;
+    if (const char *e = getenv("CHZ_LDS_CAP")) {
+        size_t c = (size_t)atol(e);
+        if (c > 0 && c < lds_cap_) lds_cap_ = c;
+    }
+}
+
+Context &Context::get()
+{
+    static std::mutex m;
+    static std::map<int, Context *> all;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    std::lock_guard<std::mutex> g(m);
+    auto it = all.find(dev);
+    if (it != all.end()) return *it->second;
+    Context *c = new Context(dev);
+    all[dev] = c;
+    return *c;
+}
+
+// Twiddles are generated in FP64 with the host libm and rounded once to the
+// pass precision (the FP32 path uses FP64-generated, rounded twiddles).
+const void *Context::twiddles(int N, int prec)
+{
+    auto key = std::make_pair(N, prec);
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second;
+    const int M = N / 2;
+    const int n = TwOffsets::total(N);
+    std::vector<double> h(2 * (size_t)n, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k <= 2 * N; k++) {
+        double a = pi * (double)k / (2.0 * N);
+        h[2 * k] = std::cos(a);
+        h[2 * k + 1] = std::sin(a);
+    }
+    for (int j = 0; j < M; j++) {
+        double a = 2.0 * pi * (double)j / (double)M;
+        h[2 * (TwOffsets::fftw(N) + j)] = std::cos(a);
+        h[2 * (TwOffsets::fftw(N) + j) + 1] = -std::sin(a);
+    }
+    for (int k = 0; k <= M; k++) {
+        double a = pi * (double)k / (double)N;
+        h[2 * (TwOffsets::rsin(N) + k)] = k == 0 ? 0.0 : 1.0 / std::sin(a);
+    }
+    void *d = nullptr;
+    size_t bytes = h.size() * (prec == 64 ? 8 : 4);
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    hipError_t e;
+    if (prec == 64) {
+        e = hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> f(h.size());
+        for (size_t i = 0; i < h.size(); i++) f[i] = (float)h[i];
+        e = hipMemcpy(d, f.data(), bytes, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    tw_[key] = d;
+    return d;
+}
+
+bool Context::config(int N, int prec, bool rows, AxisCfg *out)
+{
+    auto key = std::make_tuple(N, prec, rows);
+    auto it = cfg_.find(key);
+    if (it != cfg_.end()) {
+        *out = it->second;
+        return true;
+    }
+    const int M = N / 2;
+    static const int epts[] = {16, 12, 8, 6, 4, 3, 2, 1};
+    AxisCfg c;
+    c.N = N;
+    c.M = M;
+    for (int e : epts) {
+        if (M % e) continue;
+        if (M / e > 256) continue;
+        unsigned long long p;
+        if (!radix_plan(M, e, &p)) continue;
+        c.EPT = e;
+        c.TPL = M / e;
+        c.plan = p;
+        break;
+    }
+    if (!c.EPT) return false;
+    int L = 256 / c.TPL;
+    if (L < 1) L = 1;
+    const size_t cap = lds_cap_ < 65536 ? lds_cap_ : 65536;
+    while (L > 1 && axis_pass_lds(prec, M, L, c.TPL) > cap) L--;
+    if (axis_pass_lds(prec, M, L, c.TPL) > lds_cap_) return false;
+    c.L = L;
+    cfg_[key] = c;
+    *out = c;
+    return true;
+}
+
+void *Context::workspace(size_t bytes, int slot)
+{
+    auto it = ws_.find(slot);
+    if (it != ws_.end() && it->second.second >= bytes) return it->second.first;
+    if (it != ws_.end()) {
+        (void)hipFree(it->second.first);
+        ws_.erase(it);
+    }
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    ws_[slot] = std::make_pair(d, bytes);
+    return d;
+}
+
+void Context::release()
+{
+    for (auto &kv : tw_) (void)hipFree(kv.second);
+    tw_.clear();
+    for (auto &kv : ws_) (void)hipFree(kv.second.first);
+    ws_.clear();
+    cfg_.clear();
+}
+
+}  // namespace chz

@@ -1,0 +1,221 @@
+// chz_device.h -- device building blocks of the line-transform passes.
+//
+// One workgroup owns a tile of L lines of one axis.  A line holds the N+2
+// real slots of the padded layout (N cells, M = N/2 complex points).  In LDS
+// the tile is stored as complex points, point-major: point k of line c sits
+// at buf[k*LS + c] with LS = L + 1 (odd, so the row-wise staging of the
+// x-pass and the column-wise staging of the y/z passes are both free of bank
+// conflicts).  Real slot s of line c is component (s & 1) of point s >> 1.
+//
+// Thread t of the workgroup works on line c = t % L as sub-thread
+// u = t / L (TPL = blockDim / L sub-threads per line), so consecutive lanes
+// touch consecutive lines: every LDS access of the fold, FFT and unfold
+// phases is a contiguous run of L points.
+//
+// The FFT is a Stockham autosort over M points with each thread holding EPT
+// points in registers (M = TPL * EPT); a stage of radix R runs EPT/R
+// butterflies per thread.  Only the forward transform (e^{-2 pi i/M}) is
+// implemented; the inverse used by the C2R-type kinds is conj(FFT(conj(.))),
+// folded into the fold and unfold phases.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chz {
+
+template <typename T> struct vec2_of;
+template <> struct vec2_of<double> { typedef double2 type; };
+template <> struct vec2_of<float> { typedef float2 type; };
+
+template <typename V> __device__ __forceinline__ V mk(decltype(V().x) a, decltype(V().x) b) { V r; r.x = a; r.y = b; return r; }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) { return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename V> __device__ __forceinline__ V cconj(V a) { return mk<V>(a.x, -a.y); }
+template <typename V> __device__ __forceinline__ V cmul_mi(V a) { return mk<V>(a.y, -a.x); }   // a * (-i)
+template <typename V> __device__ __forceinline__ V cmul_pi(V a) { return mk<V>(-a.y, a.x); }   // a * (+i)
+template <typename V> __device__ __forceinline__ V cscale(V a, decltype(V().x) s) { return mk<V>(a.x * s, a.y * s); }
+
+// cos(2 pi k / 48) for any integer k, from the exact values at multiples of
+// pi/24 that the radices 2,3,4,6,8,12,16 need (k a multiple of 3 or 4).
+__host__ __device__ constexpr double cos48(int k)
+{
+    k %= 48;
+    if (k < 0) k += 48;
+    if (k > 24) k = 48 - k;               // cos(2pi - a) = cos(a)
+    bool neg = false;
+    if (k > 12) { k = 24 - k; neg = true; } // cos(pi - a) = -cos(a)
+    double v = k == 0 ? 1.0
+             : k == 3 ? 0.92387953251128675613    // cos(pi/8)
+             : k == 4 ? 0.86602540378443864676    // cos(pi/6)
+             : k == 6 ? 0.70710678118654752440    // cos(pi/4)
+             : k == 8 ? 0.5                       // cos(pi/3)
+             : k == 9 ? 0.38268343236508977173    // cos(3pi/8)
+             : 0.0;                               // k == 12
+    return neg ? -v : v;
+}
+// W_R^e = e^{-2 pi i e / R}: (cos, -sin)
+template <int R, int E> struct Root {
+    static constexpr int k = ((E % R) * (48 / R));
+    static constexpr double c = cos48(k);
+    static constexpr double s = -cos48(12 - k);   // -sin(a) = -cos(pi/2 - a)
+};
+
+template <int Begin, int End, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (Begin < End) {
+        f(std::integral_constant<int, Begin>{});
+        static_for<Begin + 1, End>(f);
+    }
+}
+
+// a * W_R^E with the trivial cases resolved at compile time
+template <int R, int E, typename V> __device__ __forceinline__ V rot(V a)
+{
+    constexpr int e = ((E % R) + R) % R;
+    if constexpr (e == 0) return a;
+    else if constexpr (2 * e == R) return mk<V>(-a.x, -a.y);
+    else if constexpr (4 * e == R) return cmul_mi(a);
+    else if constexpr (4 * e == 3 * R) return cmul_pi(a);
+    else {
+        typedef decltype(V().x) T;
+        constexpr T c = (T)Root<R, e>::c, s = (T)Root<R, e>::s;
+        return mk<V>(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// ---- forward small DFTs in registers: x[o + st*r], r = 0..R-1 -------------
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+    template <typename V> __device__ __forceinline__ static void run(V *x, int o, int st)
+    {
+        V a = x[o], b = x[o + st];
+        x[o] = cadd(a, b);
+        x[o + st] = csub(a, b);
+    }
+};
+template <> struct Dft<3> {
+    template <typename V> __device__ __forceinline__ static void run(V *x, int o, int st)
+    {
+        typedef decltype(V().x) T;
+        const T h = (T)0.86602540378443864676;   // sin(2 pi/3)
+        V a0 = x[o], a1 = x[o + st], a2 = x[o + 2 * st];
+        V t1 = cadd(a1, a2), t2 = csub(a1, a2);
+        V m = mk<V>(a0.x - (T)0.5 * t1.x, a0.y - (T)0.5 * t1.y);
+        V n = mk<V>(h * t2.y, -h * t2.x);          // -i * sin(2pi/3) * t2
+        x[o] = cadd(a0, t1);
+        x[o + st] = cadd(m, n);
+        x[o + 2 * st] = csub(m, n);
+    }
+};
+template <> struct Dft<4> {
+    template <typename V> __device__ __forceinline__ static void run(V *x, int o, int st)
+    {
+        V a0 = x[o], a1 = x[o + st], a2 = x[o + 2 * st], a3 = x[o + 3 * st];
+        V t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmul_mi(csub(a1, a3));
+        x[o] = cadd(t0, t2);
+        x[o + 2 * st] = csub(t0, t2);
+        x[o + st] = cadd(t1, t3);
+        x[o + 3 * st] = csub(t1, t3);
+    }
+};
+// R = P*Q: input index n = Q*n1 + n2, output index k = k1 + P*k2.
+// P-point DFTs over n1 for each n2, twiddle W_R^{n2 k1}, Q-point DFTs over n2.
+template <int P, int Q> struct DftPQ {
+    template <typename V> __device__ __forceinline__ static void run(V *x, int o, int st)
+    {
+        constexpr int R = P * Q;
+        V y[R];
+        static_for<0, Q>([&](auto n2c) {
+            constexpr int n2 = decltype(n2c)::value;
+            V t[P];
+            static_for<0, P>([&](auto n1c) { t[decltype(n1c)::value] = x[o + st * (Q * decltype(n1c)::value + n2)]; });
+            Dft<P>::run(t, 0, 1);
+            static_for<0, P>([&](auto k1c) {
+                constexpr int k1 = decltype(k1c)::value;
+                y[n2 * P + k1] = rot<R, n2 * k1>(t[k1]);
+            });
+        });
+        static_for<0, P>([&](auto k1c) {
+            constexpr int k1 = decltype(k1c)::value;
+            V t[Q];
+            static_for<0, Q>([&](auto n2c) { t[decltype(n2c)::value] = y[decltype(n2c)::value * P + k1]; });
+            Dft<Q>::run(t, 0, 1);
+            static_for<0, Q>([&](auto k2c) { x[o + st * (k1 + P * decltype(k2c)::value)] = t[decltype(k2c)::value]; });
+        });
+    }
+};
+template <> struct Dft<6> : DftPQ<2, 3> {};
+template <> struct Dft<8> : DftPQ<2, 4> {};
+template <> struct Dft<12> : DftPQ<4, 3> {};
+template <> struct Dft<16> : DftPQ<4, 4> {};
+
+// ---- tile addressing ---------------------------------------------------
+template <typename T> struct Tile {
+    typedef typename vec2_of<T>::type V;
+    V *buf;     // (M+1) * LS points
+    int LS;     // point stride (L + 1)
+    int c;      // line of this thread
+    __device__ __forceinline__ V &pt(int k) const { return buf[k * LS + c]; }
+    __device__ __forceinline__ T &re(int s) const { return reinterpret_cast<T *>(buf)[(size_t)((s >> 1) * LS + c) * 2 + (s & 1)]; }
+};
+
+// One Stockham stage of radix R over M = TPL*EPT points, in place.
+// Reads points u + TPL*e (e = 0..EPT-1), writes the permuted outputs.
+template <int R, int EPT, typename T>
+__device__ __forceinline__ void fft_stage(const Tile<T> &tl, int u, int TPL, int M, int Ns,
+                                          const typename vec2_of<T>::type *__restrict__ tw)
+{
+    typedef typename vec2_of<T>::type V;
+    constexpr int NB = EPT / R;
+    V v[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) v[e] = tl.pt(u + TPL * e);
+    __syncthreads();
+    const int twstride = M / (Ns * R);
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int b = u + i * TPL;
+        const int j = b % Ns;
+        V x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = v[i + r * NB];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) x[r] = cmul(x[r], tw[r * j * twstride]);
+        }
+        Dft<R>::run(x, 0, 1);
+        const int base = (b - j) * R + j;
+#pragma unroll
+        for (int q = 0; q < R; q++) tl.pt(base + q * Ns) = x[q];
+    }
+    __syncthreads();
+}
+
+// Full forward FFT: stages described by a packed radix list (one byte per
+// stage, first stage in the low byte, 0 terminates).
+template <int EPT, typename T>
+__device__ __forceinline__ void fft_forward(const Tile<T> &tl, int u, int TPL, int M, unsigned long long plan,
+                                            const typename vec2_of<T>::type *__restrict__ tw)
+{
+    int Ns = 1;
+    for (int st = 0; st < 8; st++) {
+        const int R = (int)((plan >> (8 * st)) & 255);
+        if (R == 0) break;
+        switch (R) {
+            case 2: if constexpr (EPT % 2 == 0) fft_stage<2, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 3: if constexpr (EPT % 3 == 0) fft_stage<3, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 4: if constexpr (EPT % 4 == 0) fft_stage<4, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 6: if constexpr (EPT % 6 == 0) fft_stage<6, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 8: if constexpr (EPT % 8 == 0) fft_stage<8, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 12: if constexpr (EPT % 12 == 0) fft_stage<12, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 16: if constexpr (EPT % 16 == 0) fft_stage<16, EPT>(tl, u, TPL, M, Ns, tw); break;
+            default: break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace chz

@@ -1,0 +1,50 @@
+// chz_kernels.h -- host-visible declarations shared by the kernels and the
+// host orchestration: kind ordinals, twiddle-table layout, pass arguments and
+// the launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace chz {
+
+enum Kind { K_DCT_1 = 0, K_DCT_2, K_DCT_3, K_DST_1, K_DST_2, K_DST_3, K_R2C, K_C2R };
+
+__host__ __device__ constexpr bool kind_is_z2d(int k) { return k == K_DST_1 || k == K_DCT_1 || k == K_DCT_2 || k == K_DST_2 || k == K_C2R; }
+
+// Twiddle table of one N (host-generated in FP64, rounded to T), all as V:
+//   half[n] = (cos(n pi/2N), sin(n pi/2N)),  n = 0..2N     (offset 0)
+//   fftw[j] = (cos(2 pi j/M), -sin(2 pi j/M)), j = 0..M-1   (offset 2N+1)
+//   rsin[n] = (1/sin(n pi/N), 0),            n = 0..M      (offset 2N+1+M; rsin[0] = 0)
+struct TwOffsets {
+    __host__ __device__ static int half(int) { return 0; }
+    __host__ __device__ static int fftw(int N) { return 2 * N + 1; }
+    __host__ __device__ static int rsin(int N) { return 2 * N + 1 + N / 2; }
+    __host__ __device__ static int total(int N) { return 2 * N + 1 + N / 2 + N / 2 + 1; }
+};
+
+struct PassArgs {
+    const void *src;
+    void *dst;
+    long long s_sa, s_sb, s_sp;   // source strides (elements) of line index a, b and slot
+    long long d_sa, d_sb, d_sp;   // destination strides
+    int s_rowlen, d_rowlen;       // row tiles: slots per source / destination row (= s_sa / d_sa)
+    int a0, na, b0;               // lines a in [a0, a0+na) tiled by L; b = b0 + blockIdx.y
+    int N, M, kind;
+    int in_lo, in_hi;             // slots loaded (others 0)
+    int st_lo, st_hi;             // slots stored
+    int out_lo, out_hi;           // valid output slots (others stored as 0)
+    int L, TPL;
+    unsigned long long plan;
+    const void *tw;
+    double scale;
+};
+
+__host__ __device__ inline int lds_points(int M, int L) { return (M + 1) * (L + 1); }
+
+// Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
+// FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.
+hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s);
+const void *axis_pass_kernel(int prec, int ept, bool rows);
+size_t axis_pass_lds(int prec, int M, int L, int TPL);
+
+}  // namespace chz

@@ -1,0 +1,381 @@
+// chz_pass.h -- the fused axis pass: stage a tile of lines into LDS, apply
+// the per-kind fold, the M-point FFT and the per-kind unfold, store.
+//
+// Kind conventions are the reference's (all unnormalised, 1/2 of FFTW):
+//   DST_1  p[1..N-1] -> p[1..N-1]   cudasymmfft/dst1funcinplace.cu:55-112
+//   DCT_1  p[0..N]   -> p[0..N]     cudasymmfft/dct1funcinplace.cu:60-131
+//   DCT_2  p[0..N-1] -> p[0..N-1]   cudasymmfft/dct2funcinplace.cu:49-112
+//   DST_2  p[1..N]   -> p[1..N]     cudasymmfft/dst2funcinplace.cu:48-108
+//   DCT_3  p[0..N-1] -> p[0..N-1]   cudasymmfft/dct3funcinplace.cu:47-95
+//   DST_3  p[1..N]   -> p[1..N]     cudasymmfft/dst3funcinplace.cu:49-105 (p[0] taken as 0)
+//   R2C    p[0..N-1] -> p[0..N+1]   cudasymmfft/dftfunc.cu:56-66
+//   C2R    p[0..N+1] -> p[0..N-1]   cudasymmfft/dftfunc.cu:72-82
+// The reference runs each as preOp kernel -> cuFFT Z2D/D2Z of length N ->
+// postOp kernel.  Here the real-FFT split (N real <-> N/2 complex) is merged
+// into the same fold/unfold so each kind costs exactly one M = N/2 point
+// complex FFT on data that never leaves LDS.
+#pragma once
+#include "chz_device.h"
+#include "chz_kernels.h"
+
+namespace chz {
+
+// ---------------------------------------------------------------------------
+// Fold: real line (slots) -> FFT input points 0..M-1, in place.
+// ---------------------------------------------------------------------------
+
+// Half-complex spectrum H_k (k = 0..M) that the reference's preOp builds for
+// the Z2D-type kinds, read from the real line x.
+template <int KIND, typename T>
+__device__ __forceinline__ typename vec2_of<T>::type hval(const Tile<T> &tl, int k, int M, int N)
+{
+    typedef typename vec2_of<T>::type V;
+    const T h = (T)0.5;
+    if constexpr (KIND == K_DST_1) {   // dst1funcinplace.cu:70-79
+        if (k == 0) return mk<V>((T)2 * tl.re(1), 0);
+        if (k == M) return mk<V>((T)-2 * tl.re(N - 1), 0);
+        return mk<V>(tl.re(2 * k + 1) - tl.re(2 * k - 1), -tl.re(2 * k));
+    } else if constexpr (KIND == K_DCT_1) {   // dct1funcinplace.cu:66-77 (even slots kept, odd differenced)
+        if (k == 0) return mk<V>(tl.re(0), 0);
+        if (k == M) return mk<V>(tl.re(N), 0);
+        return mk<V>(tl.re(2 * k), tl.re(2 * k - 1) - tl.re(2 * k + 1));
+    } else if constexpr (KIND == K_DCT_2) {   // dct2funcinplace.cu:63-71
+        if (k == 0) return mk<V>(tl.re(0), 0);
+        if (k == M) return mk<V>(tl.re(N - 1), 0);
+        return mk<V>(h * (tl.re(2 * k) + tl.re(2 * k - 1)), h * (tl.re(2 * k) - tl.re(2 * k - 1)));
+    } else if constexpr (KIND == K_DST_2) {   // dst2funcinplace.cu:55-69 (line read from p[1..N])
+        if (k == 0) return mk<V>(tl.re(1), 0);
+        if (k == M) return mk<V>(-tl.re(N), 0);
+        return mk<V>(h * (tl.re(2 * k + 1) - tl.re(2 * k)), -h * (tl.re(2 * k + 1) + tl.re(2 * k)));
+    } else {                                  // K_C2R: imaginary parts of DC and Nyquist do not enter
+        if (k == 0) return mk<V>(tl.re(0), 0);
+        if (k == M) return mk<V>(tl.re(N), 0);
+        const V v = tl.pt(k);
+        return mk<V>(v.x, v.y);
+    }
+}
+
+template <int KIND, int EPT, typename T>
+__device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, int N,
+                                     const typename vec2_of<T>::type *__restrict__ tw, T *red)
+{
+    typedef typename vec2_of<T>::type V;
+    constexpr int PP = EPT / 2 + 1;        // >= ceil((M/2 + 1)/TPL)
+    constexpr int PT = EPT + 1;            // >= ceil((M + 1)/TPL)
+    if constexpr (kind_is_z2d(KIND)) {
+        // Z2D via an M-point inverse FFT: C_k = E_k + i O_k,
+        // E_k = H_k + conj H_{M-k}, O_k = (H_k - conj H_{M-k}) w^k, w = e^{2 pi i/N};
+        // C_{M-k} = conj E_k + i conj O_k.  Stored conjugated (inverse = conj o FFT o conj).
+        V ca[PP], cb[PP];
+        if constexpr (KIND == K_DCT_1) {   // x1 = sum of odd samples (dct1funcinplace.cu:86-93)
+            T s = 0;
+            for (int j = u; j < M; j += TPL) s += tl.re(2 * j + 1);
+            red[u * (tl.LS - 1) + tl.c] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            if (k <= M / 2) {
+                V Hk = hval<KIND, T>(tl, k, M, N), Hm = hval<KIND, T>(tl, M - k, M, N);
+                V E = mk<V>(Hk.x + Hm.x, Hk.y - Hm.y);
+                V D = mk<V>(Hk.x - Hm.x, Hk.y + Hm.y);
+                V O = cmul(D, tw[4 * k]);
+                ca[i] = mk<V>(E.x - O.y, -(E.y + O.x));
+                cb[i] = mk<V>(E.x + O.y, E.y - O.x);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            if (k <= M / 2) {
+                tl.pt(k) = ca[i];
+                if (k != 0 && 2 * k != M) tl.pt(M - k) = cb[i];
+            }
+        }
+    } else if constexpr (KIND != K_R2C) {
+        // DCT_3 / DST_3 pre-rotation of the pairs (t, N-t), then the real line
+        // itself is the FFT input (point m = x'_{2m} + i x'_{2m+1}).
+        T va[PT], vb[PT];
+        const V *half = tw;
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int t = u + i * TPL;
+            if (t <= M) {
+                const V cs = half[t];      // (cos, sin)(t pi / 2N)
+                if constexpr (KIND == K_DCT_3) {   // dct3funcinplace.cu:53-64 (its t+1 is t here)
+                    if (t >= 1) {
+                        const T a = tl.re(t), b = tl.re(N - t);
+                        const T Ta = a + b, Tb = a - b;
+                        va[i] = Ta * cs.y + Tb * cs.x;
+                        vb[i] = Ta * cs.x - Tb * cs.y;
+                    }
+                } else {                   // DST_3, dst3funcinplace.cu:55-65 with p[0] = 0
+                    const T a = t == 0 ? (T)0 : tl.re(t), b = tl.re(N - t);
+                    const T Ta = a + b, Tb = a - b;
+                    va[i] = Ta * cs.x + Tb * cs.y;
+                    vb[i] = Ta * cs.y - Tb * cs.x;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int t = u + i * TPL;
+            if (t <= M) {
+                if constexpr (KIND == K_DCT_3) {
+                    if (t >= 1) {
+                        tl.re(t) = va[i];
+                        if (2 * t != N) tl.re(N - t) = vb[i];
+                    }
+                } else {
+                    tl.re(t) = va[i];
+                    if (t != 0 && 2 * t != N) tl.re(N - t) = vb[i];
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Unfold: FFT output points -> real output slots, in place.
+// ---------------------------------------------------------------------------
+template <int KIND, int EPT, typename T>
+__device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M, int N,
+                                       const typename vec2_of<T>::type *__restrict__ tw, const T *red)
+{
+    typedef typename vec2_of<T>::type V;
+    constexpr int PP = EPT / 2 + 1;
+    constexpr int PT = EPT + 1;
+    const T h = (T)0.5;
+    if constexpr (kind_is_z2d(KIND)) {
+        // y_n (n = 0..N-1) is the real view of conj(FFT): y_{2m} = re, y_{2m+1} = -im.
+        T oa[PT], ob[PT];
+        T x1 = 0;
+        if constexpr (KIND == K_DCT_1) {
+            const int L = tl.LS - 1;
+            for (int v = 0; v < TPL; v++) x1 += red[v * L + tl.c];
+        }
+        const V *half = tw;
+        const V *rsin = tw + TwOffsets::rsin(N);
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int n = u + i * TPL;
+            if (n <= M) {
+                const T a = (n & 1) ? -tl.re(n) : tl.re(n);
+                const T b = n == 0 ? (T)0 : (((N - n) & 1) ? -tl.re(N - n) : tl.re(N - n));
+                if constexpr (KIND == K_DST_1) {          // dst1funcinplace.cu:97-107
+                    if (n == 0) { oa[i] = 0; ob[i] = 0; }
+                    else {
+                        const T Ta = (a + b) * rsin[n].x * (T)0.25, Tb = (a - b) * h;
+                        oa[i] = (Ta + Tb) * h; ob[i] = (Ta - Tb) * h;
+                    }
+                } else if constexpr (KIND == K_DCT_1) {   // dct1funcinplace.cu:116-127
+                    if (n == 0) { oa[i] = a * h + x1; ob[i] = a * h - x1; }
+                    else {
+                        const T Ta = (a + b) * h, Tb = (a - b) * (T)0.25 * rsin[n].x;
+                        oa[i] = (Ta - Tb) * h; ob[i] = (Ta + Tb) * h;
+                    }
+                } else if constexpr (KIND == K_DCT_2) {   // dct2funcinplace.cu:93-105 (out[0] = y_0)
+                    if (n == 0) { oa[i] = a; ob[i] = 0; }
+                    else {
+                        const V cs = half[n];
+                        const T Ta = a + b, Tb = a - b;
+                        oa[i] = (Ta * cs.x + Tb * cs.y) * h; ob[i] = (Ta * cs.y - Tb * cs.x) * h;
+                    }
+                } else if constexpr (KIND == K_DST_2) {   // dst2funcinplace.cu:89-103 (out[0] = 0, out[N] = y_0)
+                    if (n == 0) { oa[i] = 0; ob[i] = a; }
+                    else {
+                        const V cs = half[n];
+                        const T Ta = a + b, Tb = a - b;
+                        oa[i] = (Ta * cs.y + Tb * cs.x) * h; ob[i] = (Ta * cs.x - Tb * cs.y) * h;
+                    }
+                } else {                                  // K_C2R
+                    oa[i] = a; ob[i] = b;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int n = u + i * TPL;
+            if (n <= M) {
+                if (n == 0) {
+                    tl.re(0) = oa[i];
+                    if constexpr (KIND == K_DCT_1 || KIND == K_DST_2 || KIND == K_DST_1) tl.re(N) = ob[i];
+                } else {
+                    tl.re(n) = oa[i];
+                    if (2 * n != N) tl.re(N - n) = ob[i];
+                }
+            }
+        }
+    } else {
+        // D2Z split of the M-point FFT Z into the N-point spectrum X_k (k = 0..M):
+        // E_k = (Z_k + conj Z_{M-k})/2, O_k = -i (Z_k - conj Z_{M-k})/2,
+        // X_k = E_k + conj(w^k) O_k, X_{M-k} = conj(E_k - conj(w^k) O_k).
+        V xa[PP], xb[PP];
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            if (k <= M / 2) {
+                const V Zk = tl.pt(k), Zm = tl.pt(k == 0 ? 0 : M - k);
+                const V E = mk<V>(h * (Zk.x + Zm.x), h * (Zk.y - Zm.y));
+                const V O = mk<V>(h * (Zk.y + Zm.y), -h * (Zk.x - Zm.x));
+                const V w = tw[4 * k];
+                const V P = mk<V>(w.x * O.x + w.y * O.y, w.x * O.y - w.y * O.x);   // conj(w) O
+                xa[i] = mk<V>(E.x + P.x, E.y + P.y);
+                xb[i] = mk<V>(E.x - P.x, -(E.y - P.y));
+            }
+        }
+        __syncthreads();
+        auto put = [&](int t, V X) {
+            if constexpr (KIND == K_R2C) {            // interleaved Re/Im, Im of DC and Nyquist exactly 0
+                tl.re(2 * t) = X.x;
+                tl.re(2 * t + 1) = (t == 0 || t == M) ? (T)0 : X.y;
+            } else if constexpr (KIND == K_DCT_3) {   // dct3funcinplace.cu:83-91
+                if (t == 0) { tl.re(0) = X.x * h; }
+                else {
+                    tl.re(2 * t - 1) = (X.x - X.y) * h;
+                    if (2 * t < N) tl.re(2 * t) = (X.x + X.y) * h;
+                }
+            } else {                                  // K_DST_3, dst3funcinplace.cu:84-101
+                if (t == 0) { tl.re(0) = 0; tl.re(1) = X.x * h; }
+                else {
+                    tl.re(2 * t) = -(X.y + X.x) * h;
+                    if (2 * t + 1 <= N) tl.re(2 * t + 1) = (X.x - X.y) * h;
+                }
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            if (k <= M / 2) {
+                put(k, xa[i]);
+                if (2 * k != M) put(M - k, xb[i]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Pass description and the generic kernel body
+// ---------------------------------------------------------------------------
+
+
+
+// Load the tile into LDS.  ROWS: lines are consecutive rows of a row-major
+// volume (x-pass), the tile is one contiguous run of memory.  !ROWS: lines
+// are L consecutive x-columns (y/z passes), slot stride s_sp.
+template <typename T, bool ROWS, typename SRC>
+__device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, const SRC *src)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int NS = p.N + 2;
+    const int L = p.L;
+    if (ROWS) {
+        const int RL = p.s_rowlen;
+        const int total = nlines * RL;
+        const SRC *base = src + lbase;
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+        const int LS = tl.LS;
+        // incremental (row, slot) of idx = t + j*nt
+        int r = t / RL, s = t - (t / RL) * RL;
+        const int dq = nt / RL, dr = nt - (nt / RL) * RL;
+        for (int idx = t; idx < total; idx += nt) {
+            if (s < NS) {
+                T v = 0;
+                if (s >= p.in_lo && s <= p.in_hi) v = (T)base[idx];
+                rbuf[(size_t)((s >> 1) * LS + r) * 2 + (s & 1)] = v;
+            }
+            s += dr; r += dq;
+            if (s >= RL) { s -= RL; r++; }
+        }
+        // slots beyond the source row, and lines beyond the tile end, are 0
+        if (RL < NS) {
+            const int extra = NS - RL;
+            for (int idx = t; idx < nlines * extra; idx += nt) {
+                const int rr = idx / extra, ss = RL + idx - (idx / extra) * extra;
+                rbuf[(size_t)((ss >> 1) * LS + rr) * 2 + (ss & 1)] = 0;
+            }
+        }
+        for (int idx = t; idx < (L - nlines) * NS; idx += nt) {
+            const int rr = nlines + idx / NS, ss = idx - (idx / NS) * NS;
+            rbuf[(size_t)((ss >> 1) * LS + rr) * 2 + (ss & 1)] = 0;
+        }
+    } else {
+        const int c = t % L, u0 = t / L, step = nt / L;
+        const bool ok = c < nlines;
+        const SRC *base = src + lbase + (long long)c * p.s_sa;
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+        for (int s = u0; s < NS; s += step) {
+            T v = 0;
+            if (ok && s >= p.in_lo && s <= p.in_hi) v = (T)base[(long long)s * p.s_sp];
+            rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)] = v;
+        }
+    }
+}
+
+template <typename T, bool ROWS, typename DST>
+__device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, DST *dst)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    const T sc = (T)p.scale;
+    const T *rbuf = reinterpret_cast<const T *>(tl.buf);
+    if (ROWS) {
+        const int RL = p.d_rowlen;
+        const int total = nlines * RL;
+        DST *base = dst + lbase;
+        int r = t / RL, s = t - (t / RL) * RL;
+        const int dq = nt / RL, dr = nt - (nt / RL) * RL;
+        for (int idx = t; idx < total; idx += nt) {
+            if (s >= p.st_lo && s <= p.st_hi) {
+                T v = 0;
+                if (s >= p.out_lo && s <= p.out_hi) v = rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] * sc;
+                base[idx] = (DST)v;
+            }
+            s += dr; r += dq;
+            if (s >= RL) { s -= RL; r++; }
+        }
+    } else {
+        const int L = p.L;
+        const int c = t % L, u0 = t / L, step = nt / L;
+        if (c >= nlines) return;
+        DST *base = dst + lbase + (long long)c * p.d_sa;
+        for (int s = p.st_lo + u0; s <= p.st_hi; s += step) {
+            T v = 0;
+            if (s >= p.out_lo && s <= p.out_hi) v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)] * sc;
+            base[(long long)s * p.d_sp] = (DST)v;
+        }
+    }
+}
+
+// One line transform of the whole tile (fold, FFT, unfold), data in LDS.
+template <int KIND, int EPT, typename T>
+__device__ __forceinline__ void line_transform_k(const Tile<T> &tl, int u, int TPL, int M, int N,
+                                                 unsigned long long plan, const typename vec2_of<T>::type *tw, T *red)
+{
+    fold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
+    fft_forward<EPT, T>(tl, u, TPL, M, plan, tw + TwOffsets::fftw(N));
+    unfold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
+}
+
+// Runtime kind -> one specialised path (the branch is uniform over the workgroup).
+template <int EPT, typename T>
+__device__ __forceinline__ void line_transform(int kind, const Tile<T> &tl, int u, int TPL, int M, int N,
+                                               unsigned long long plan, const typename vec2_of<T>::type *tw, T *red)
+{
+    switch (kind) {
+        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DST_1: line_transform_k<K_DST_1, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DST_2: line_transform_k<K_DST_2, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DST_3: line_transform_k<K_DST_3, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_R2C: line_transform_k<K_R2C, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        default: line_transform_k<K_C2R, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+    }
+}
+
+}  // namespace chz

@@ -1,0 +1,205 @@
+// chz_transform.cpp -- 3D transform orchestration (do_transform) and its C ABI.
+//
+// The reference runs, per axis, a preOp kernel, a cuFFT call, a postOp kernel
+// and a cuTranspose 3D permutation, with a device sync after each axis
+// (cudasymmfft/funcinterface.cu:200-334).  Here each axis is ONE fused pass
+// that reads its lines where they lie in the padded volume (x: contiguous
+// rows; y, z: coalesced column tiles), so a transform is 3 read+write sweeps
+// with no transposes, all stream-ordered.
+#include <cstdio>
+#include <string>
+
+#include "chz_context.h"
+#include "chz_transform.h"
+
+namespace chz {
+
+void in_window(int kind, int N, int *lo, int *hi)
+{
+    switch (kind) {
+        case K_DST_1: *lo = 1; *hi = N - 1; return;
+        case K_DCT_1: *lo = 0; *hi = N; return;
+        case K_DCT_2: *lo = 0; *hi = N - 1; return;
+        case K_DST_2: *lo = 1; *hi = N; return;
+        case K_DCT_3: *lo = 0; *hi = N - 1; return;
+        case K_DST_3: *lo = 1; *hi = N; return;
+        case K_R2C: *lo = 0; *hi = N - 1; return;
+        default: *lo = 0; *hi = N + 1; return;   // K_C2R
+    }
+}
+
+void out_window(int kind, int N, int *lo, int *hi)
+{
+    switch (kind) {
+        case K_R2C: *lo = 0; *hi = N + 1; return;
+        case K_C2R: *lo = 0; *hi = N - 1; return;
+        default: in_window(kind, N, lo, hi); return;
+    }
+}
+
+int check_axis(int N, int kind, const char *axis)
+{
+    if (kind < 0 || kind > 7) return fail(ERR_INVALID, std::string("invalid transform kind for ") + axis);
+    if (N < 2 || (N & 1)) return fail(ERR_INVALID, std::string("cell count must be even and >= 2 along ") + axis);
+    return OK;
+}
+
+int fill_pass(PassArgs &p, const AxisCfg &c, int prec, int kind, const void *tw)
+{
+    p.N = c.N;
+    p.M = c.M;
+    p.kind = kind;
+    p.L = c.L;
+    p.TPL = c.TPL;
+    p.plan = c.plan;
+    p.tw = tw;
+    p.scale = 1.0;
+    (void)prec;
+    return OK;
+}
+
+// Host-side proof that a pass stays inside its buffers: the lowest and
+// highest element each load / store of the pass can touch.
+bool pass_in_bounds(const PassArgs &p, bool rows, int nb, long long src_elems, long long dst_elems)
+{
+    if (p.na <= 0 || nb <= 0 || p.a0 < 0 || p.b0 < 0 || p.L <= 0 || p.TPL <= 0) return false;
+    if (p.in_lo < 0 || p.in_hi > p.N + 1 || p.st_lo < 0 || p.st_hi > p.N + 1) return false;
+    const long long a1 = p.a0 + p.na - 1, b1 = p.b0 + nb - 1;
+    long long smax, dmax;
+    if (rows) {
+        if (p.s_sa != p.s_rowlen || p.d_sa != p.d_rowlen || p.s_sp != 1 || p.d_sp != 1) return false;
+        smax = a1 * p.s_sa + b1 * p.s_sb + p.s_rowlen - 1;
+        dmax = a1 * p.d_sa + b1 * p.d_sb + p.d_rowlen - 1;
+    } else {
+        smax = a1 * p.s_sa + b1 * p.s_sb + (long long)p.in_hi * p.s_sp;
+        dmax = a1 * p.d_sa + b1 * p.d_sb + (long long)p.st_hi * p.d_sp;
+    }
+    return smax < src_elems && dmax < dst_elems;
+}
+
+int launch_checked(const PassArgs &p, int prec, int ept, bool rows, int nb, long long src_elems, long long dst_elems,
+                   hipStream_t s, const char *what)
+{
+    if (!pass_in_bounds(p, rows, nb, src_elems, dst_elems))
+        return fail(ERR_INVALID, std::string(what) + ": pass geometry outside the buffers");
+    hipError_t e = launch_axis_pass(p, prec, ept, rows, nb, s);
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string(what) + " launch: " + hipGetErrorString(e));
+    return OK;
+}
+
+template <typename T>
+int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, int kz, hipStream_t s)
+{
+    const int prec = sizeof(T) == 8 ? 64 : 32;
+    int rc;
+    if ((rc = check_axis(NX, kx, "x")) || (rc = check_axis(NY, ky, "y")) || (rc = check_axis(NZ, kz, "z"))) return rc;
+    if (!src || !dst) return fail(ERR_INVALID, "null device pointer");
+    Context &ctx = Context::get();
+    std::lock_guard<std::mutex> g(ctx.mutex());
+    AxisCfg cx, cy, cz;
+    if (!ctx.config(NX, prec, true, &cx) || !ctx.config(NY, prec, false, &cy) || !ctx.config(NZ, prec, false, &cz))
+        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s and 3s, N <= 4096)");
+    const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
+    if (!twx || !twy || !twz) return fail(ERR_ALLOC, "twiddle table allocation failed");
+
+    const long long P0 = NX + 2, P1 = NY + 2;
+    const long long sy = P0, sz = P0 * P1;
+    int ixl, ixh, iyl, iyh, izl, izh, oxl, oxh, oyl, oyh, ozl, ozh;
+    in_window(kx, NX, &ixl, &ixh);
+    in_window(ky, NY, &iyl, &iyh);
+    in_window(kz, NZ, &izl, &izh);
+    out_window(kx, NX, &oxl, &oxh);
+    out_window(ky, NY, &oyl, &oyh);
+    out_window(kz, NZ, &ozl, &ozh);
+    const long long vol = P0 * P1 * (long long)(NZ + 2);
+
+    // X pass: rows (y, z), y in in-window(Y), z in in-window(Z); src -> dst
+    {
+        PassArgs p{};
+        fill_pass(p, cx, prec, kx, twx);
+        p.src = src;
+        p.dst = dst;
+        p.s_sa = sy; p.s_sb = sz; p.s_sp = 1;
+        p.d_sa = sy; p.d_sb = sz; p.d_sp = 1;
+        p.s_rowlen = p.d_rowlen = (int)P0;
+        p.a0 = iyl; p.na = iyh - iyl + 1; p.b0 = izl;
+        p.in_lo = ixl; p.in_hi = ixh;
+        p.st_lo = 0; p.st_hi = NX + 1;
+        p.out_lo = oxl; p.out_hi = oxh;
+        if ((rc = launch_checked(p, prec, cx.EPT, true, izh - izl + 1, vol, vol, s, "x pass"))) return rc;
+    }
+    // Y pass: columns x in out-window(X), z in in-window(Z); in place on dst
+    {
+        PassArgs p{};
+        fill_pass(p, cy, prec, ky, twy);
+        p.src = dst;
+        p.dst = dst;
+        p.s_sa = 1; p.s_sb = sz; p.s_sp = sy;
+        p.d_sa = 1; p.d_sb = sz; p.d_sp = sy;
+        p.a0 = oxl; p.na = oxh - oxl + 1; p.b0 = izl;
+        p.in_lo = iyl; p.in_hi = iyh;
+        p.st_lo = 0; p.st_hi = NY + 1;
+        p.out_lo = oyl; p.out_hi = oyh;
+        if ((rc = launch_checked(p, prec, cy.EPT, false, izh - izl + 1, vol, vol, s, "y pass"))) return rc;
+    }
+    // Z pass: columns x in out-window(X), y in out-window(Y); in place on dst
+    {
+        PassArgs p{};
+        fill_pass(p, cz, prec, kz, twz);
+        p.src = dst;
+        p.dst = dst;
+        p.s_sa = 1; p.s_sb = sy; p.s_sp = sz;
+        p.d_sa = 1; p.d_sb = sy; p.d_sp = sz;
+        p.a0 = oxl; p.na = oxh - oxl + 1; p.b0 = oyl;
+        p.in_lo = izl; p.in_hi = izh;
+        p.st_lo = 0; p.st_hi = NZ + 1;
+        p.out_lo = ozl; p.out_hi = ozh;
+        if ((rc = launch_checked(p, prec, cz.EPT, false, oyh - oyl + 1, vol, vol, s, "z pass"))) return rc;
+    }
+    return OK;
+}
+
+template int transform3d<double>(const double *, double *, int, int, int, int, int, int, hipStream_t);
+template int transform3d<float>(const float *, float *, int, int, int, int, int, int, hipStream_t);
+
+}  // namespace chz
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int chz_do_transform(double *d_data, double *d_out, int NX, int NY, int NZ, int x_kind, int y_kind, int z_kind,
+                     void *stream)
+{
+    double *dst = d_out ? d_out : d_data;
+    return chz::transform3d<double>(d_data, dst, NX, NY, NZ, x_kind, y_kind, z_kind, (hipStream_t)stream);
+}
+
+int chz_do_transform_f32(float *d_data, float *d_out, int NX, int NY, int NZ, int x_kind, int y_kind, int z_kind,
+                         void *stream)
+{
+    float *dst = d_out ? d_out : d_data;
+    return chz::transform3d<float>(d_data, dst, NX, NY, NZ, x_kind, y_kind, z_kind, (hipStream_t)stream);
+}
+
+const char *chz_last_error(void) { return chz::last_error(); }
+
+void chz_release(void) { chz::Context::get().release(); }
+
+int chz_axis_config(int N, int prec, int rows, int *ept, int *tpl, int *lines, unsigned long long *plan)
+{
+    chz::Context &ctx = chz::Context::get();
+    std::lock_guard<std::mutex> g(ctx.mutex());
+    chz::AxisCfg c;
+    if (!ctx.config(N, prec, rows != 0, &c)) return chz::fail(chz::ERR_UNSUPPORTED, "unsupported cell count");
+    if (ept) *ept = c.EPT;
+    if (tpl) *tpl = c.TPL;
+    if (lines) *lines = c.L;
+    if (plan) *plan = c.plan;
+    return chz::OK;
+}
+
+int chz_version(void) { return 100; }
+
+}  // extern "C"
