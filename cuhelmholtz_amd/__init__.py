@@ -44,6 +44,8 @@ def lib():
         L.chz_do_transform.argtypes = [vp, vp, i, i, i, i, i, i, vp]
         L.chz_do_transform_f32.argtypes = [vp, vp, i, i, i, i, i, i, vp]
         L.chz_last_error.restype = ctypes.c_char_p
+        L.chz_set_timing.argtypes = [i]
+        L.chz_get_timing.argtypes = [ctypes.POINTER(dp), i, ctypes.c_char_p, i]
         L.chz_axis_config.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                       ctypes.POINTER(ctypes.c_ulonglong)]
         _lib = L
@@ -106,6 +108,64 @@ def do_transform(d_data, d_out=None, kinds=(DCT_2, DCT_2, DCT_2)):
     _check(fn(ctypes.c_void_p(d_data.data_ptr()), ctypes.c_void_p(out.data_ptr()), NX, NY, NZ,
               int(kinds[0]), int(kinds[1]), int(kinds[2]), _stream(d_data)))
     return out
+
+
+def _faces(bcs, n, bc, dev):
+    """Validate / convert the six Neumann face arrays (None where unused)."""
+    import torch
+    NX, NY, NZ = n
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    out, keep = [], []
+    for i, b in enumerate(bcs):
+        a = i // 2
+        need = bc[a] in ((3, 4) if i % 2 == 0 else (2, 3))
+        if b is None:
+            if need:
+                raise ValueError(f"boundary code {bc[a]} along axis {a} needs face array {i}")
+            out.append(None)
+            continue
+        t = b if isinstance(b, torch.Tensor) else torch.as_tensor(b)
+        t = t.to(dev, torch.float64).contiguous().reshape(-1)
+        if t.numel() != sizes[i]:
+            raise ValueError(f"face array {i} must have {sizes[i]} values")
+        keep.append(t)
+        out.append(ctypes.c_void_p(t.data_ptr()))
+    return out, keep
+
+
+def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
+    """In-place FFT-diagonalised solve of laplace u + lam u = f on a CUDA float64
+    node tensor f shaped (NZ+1, NY+1, NX+1) (cuda_helmholtz_solver semantics).
+    bcs = (bcl, bcr, bcb, bct, bcf, bce) Neumann face arrays or None."""
+    import torch
+    NX, NY, NZ = (int(v) for v in n)
+    if not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous():
+        raise ValueError("f must be a contiguous CUDA float64 tensor")
+    if tuple(f.shape) != (NZ + 1, NY + 1, NX + 1):
+        raise ValueError("f must be shaped (NZ+1, NY+1, NX+1)")
+    faces, keep = _faces(bcs, (NX, NY, NZ), bc, f.device)
+    args = []
+    for a, N in enumerate((NX, NY, NZ)):
+        args += [float(dom[2 * a]), float(dom[2 * a + 1]), N, int(bc[a]), faces[2 * a], faces[2 * a + 1]]
+    _check(lib().chz_helmholtz_solve(*args, float(lam), ctypes.c_void_p(f.data_ptr()), _stream(f)))
+    del keep
+    return f
+
+
+def set_timing(on):
+    """Record per-pass events on every transform / solve (reference TIME_TEST counterpart)."""
+    lib().chz_set_timing(1 if on else 0)
+
+
+def get_timing(max_passes=4096):
+    """[(pass name, ms)] of every pass recorded since timing was switched on or
+    last read (waits for the events, then clears them)."""
+    ms = (ctypes.c_double * max_passes)()
+    names = ctypes.create_string_buffer(16 * max_passes)
+    n = lib().chz_get_timing(ms, max_passes, names, 16 * max_passes)
+    if n < 0:
+        _check(n)
+    return list(zip(names.value.decode().split(",")[:n], [ms[i] for i in range(n)]))
 
 
 def axis_config(N, prec=64, rows=True):

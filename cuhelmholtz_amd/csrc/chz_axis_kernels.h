@@ -1,0 +1,176 @@
+// chz_axis_kernels.h -- __global__ kernel templates; instantiated per
+// precision in the chz_k_*.hip translation units (compiled in parallel).
+#pragma once
+#include "chz_kernels.h"
+#include "chz_pass.h"
+
+namespace chz {
+
+// One fused axis pass over a tile of L lines: load -> fold -> FFT -> unfold -> store.
+template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
+__global__ void __launch_bounds__(256) k_axis_pass(PassArgs p)
+{
+    typedef typename vec2_of<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *buf = reinterpret_cast<V *>(smem);
+    const int L = p.L;
+    const Tile<T> tl{buf, L + 1, (int)threadIdx.x % L};
+    const int u = (int)threadIdx.x / L;
+    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
+    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    const int nlines = min(L, p.a0 + p.na - a_tile);
+    const long long b = p.b0 + (long long)blockIdx.y;
+    load_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
+    __syncthreads();
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, reinterpret_cast<const V *>(p.tw), red);
+    store_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
+}
+
+// Boundary folding of the reference (cudahelmholtz/_cuda_helmholtz_.inl:70-227),
+// applied to the rows (j, k) of the unknown box just loaded into LDS:
+// Dirichlet  f[first] -= f[0]/h^2,   f[last] -= f[N]/h^2
+// Neumann    f[first] += 2 g_l / h,  f[last] -= 2 g_r / h
+// in the reference's order x, y, z (cuda_helmholtz_solver.cu:234-285).
+template <typename T>
+__device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<T> &tl, int nlines, int j0, int k)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int L = tl.LS - 1;
+    const int c = t % L, u0 = t / L, step = nt / L;
+    if (c >= nlines) return;
+    const int j = j0 + c;
+    const long long fx = sa.NX + 1, fy = sa.NY + 1;
+    const double *row = sa.f + (long long)j * fx + (long long)k * fx * fy;
+    T *rbuf = reinterpret_cast<T *>(tl.buf);
+    const int xs = sa.s[0], xe = sa.e[0];
+    const bool ylo = sa.bc[1] != 0 && j == sa.s[1], yhi = sa.bc[1] != 0 && j == sa.e[1];
+    const bool zlo = sa.bc[2] != 0 && k == sa.s[2], zhi = sa.bc[2] != 0 && k == sa.e[2];
+    const bool xb = sa.bc[0] != 0;
+    if (!(xb || ylo || yhi || zlo || zhi)) return;
+    for (int s = xs + u0; s <= xe; s += step) {
+        if (!(xb && (s == xs || s == xe)) && !(ylo || yhi || zlo || zhi)) continue;
+        T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
+        double acc = (double)v;
+        if (xb && s == xs) {
+            if (sa.bc[0] == 1 || sa.bc[0] == 2) acc -= row[0] * sa.ih2[0];
+            else acc += 2 * sa.bcl[(long long)k * fy + j] * sa.ih[0];
+        }
+        if (xb && s == xe) {
+            if (sa.bc[0] == 1 || sa.bc[0] == 4) acc -= row[sa.NX] * sa.ih2[0];
+            else acc -= 2 * sa.bcr[(long long)k * fy + j] * sa.ih[0];
+        }
+        if (ylo) {
+            if (sa.bc[1] == 1 || sa.bc[1] == 2) acc -= sa.f[s + (long long)k * fx * fy] * sa.ih2[1];
+            else acc += 2 * sa.bcb[(long long)k * fx + s] * sa.ih[1];
+        }
+        if (yhi) {
+            if (sa.bc[1] == 1 || sa.bc[1] == 4) acc -= sa.f[s + (long long)sa.NY * fx + (long long)k * fx * fy] * sa.ih2[1];
+            else acc -= 2 * sa.bct[(long long)k * fx + s] * sa.ih[1];
+        }
+        if (zlo) {
+            if (sa.bc[2] == 1 || sa.bc[2] == 2) acc -= sa.f[s + (long long)j * fx] * sa.ih2[2];
+            else acc += 2 * sa.bcf[(long long)j * fx + s] * sa.ih[2];
+        }
+        if (zhi) {
+            if (sa.bc[2] == 1 || sa.bc[2] == 4) acc -= sa.f[s + (long long)j * fx + (long long)sa.NZ * fx * fy] * sa.ih2[2];
+            else acc -= 2 * sa.bce[(long long)j * fx + s] * sa.ih[2];
+        }
+        v = (T)acc;
+    }
+}
+
+// Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
+// folds, forward x transform, store into the padded spectral workspace.
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_solve_xfwd(PassArgs p, SolveArgs sa)
+{
+    typedef typename vec2_of<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *buf = reinterpret_cast<V *>(smem);
+    const int L = p.L;
+    const Tile<T> tl{buf, L + 1, (int)threadIdx.x % L};
+    const int u = (int)threadIdx.x / L;
+    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
+    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    const int nlines = min(L, p.a0 + p.na - a_tile);
+    const int k = p.b0 + (int)blockIdx.y;
+    load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
+    __syncthreads();
+    fold_boundaries<T>(sa, tl, nlines, a_tile, k);
+    __syncthreads();
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, reinterpret_cast<const V *>(p.tw), red);
+    store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+}
+
+// Solve pass 3: z lines (x, y) of the spectral box: forward z transform,
+// multiply by 1/(eig_x + eig_y + eig_z + lambda) (the reference's
+// _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
+// inverse z transform, all in LDS.
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_solve_zmid(PassArgs p, SolveArgs sa)
+{
+    typedef typename vec2_of<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *buf = reinterpret_cast<V *>(smem);
+    const int L = p.L;
+    const int c = (int)threadIdx.x % L;
+    const Tile<T> tl{buf, L + 1, c};
+    const int u = (int)threadIdx.x / L;
+    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
+    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    const int nlines = min(L, p.a0 + p.na - a_tile);
+    const int y = p.b0 + (int)blockIdx.y;
+    const V *tw = reinterpret_cast<const V *>(p.tw);
+    load_tile<T, false>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
+    __syncthreads();
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, red);
+    {
+        const int x = a_tile + c;
+        const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
+        const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
+        const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+        for (int s = p.out_lo + u; s <= p.out_hi; s += p.TPL) {
+            T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
+            const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
+            const double r = m0 ? 0.0 : 1.0 / (exy + ez[s] + sa.lambda);
+            v = (T)((double)v * r);
+        }
+    }
+    __syncthreads();
+    line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, red);
+    PassArgs q = p;
+    q.out_lo = sa.out2_lo;
+    q.out_hi = sa.out2_hi;
+    store_tile<T, false>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
+}
+
+// Periodic duplicate node N <- node 0 along one axis over the whole face
+// (cudahelmholtz/_cuda_helmholtz_.inl:58-68, 116-126, 173-183).
+template <typename T>
+__global__ void __launch_bounds__(256) k_periodic_copy(T *f, int NX, int NY, int NZ, int axis)
+{
+    const long long fx = NX + 1, fy = NY + 1;
+    const int n1 = axis == 0 ? NY + 1 : NX + 1;
+    const int n2 = axis == 2 ? NY + 1 : NZ + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n1 * n2) return;
+    const int a = (int)(idx % n1), b = (int)(idx / n1);
+    if (axis == 0) f[NX + a * fx + b * fx * fy] = f[a * fx + b * fx * fy];
+    else if (axis == 1) f[a + NY * fx + b * fx * fy] = f[a + b * fx * fy];
+    else f[a + b * fx + NZ * fx * fy] = f[a + b * fx];
+}
+
+#define CHZ_EPT_SWITCH(ept, EXPR_TEMPLATE)  \
+    switch (ept) {                          \
+        case 4: return EXPR_TEMPLATE(4);    \
+        case 5: return EXPR_TEMPLATE(5);    \
+        case 6: return EXPR_TEMPLATE(6);    \
+        case 8: return EXPR_TEMPLATE(8);    \
+        case 10: return EXPR_TEMPLATE(10);  \
+        case 12: return EXPR_TEMPLATE(12);  \
+        case 16: return EXPR_TEMPLATE(16);  \
+    }                                       \
+    return nullptr;
+
+}  // namespace chz

@@ -18,15 +18,22 @@ int fail(int code, const std::string &msg)
 }
 const char *last_error() { return g_err.c_str(); }
 
-bool radix_plan(int M, int ept, unsigned long long *plan)
+// Radix sequence for an M-point FFT with TPL threads per line and at most ept
+// points per thread: every stage R must satisfy ceil(M/(R*TPL)) <= ept/R.
+bool radix_plan(int M, int ept, int tpl, unsigned long long *plan)
 {
-    static const int radices[] = {16, 12, 8, 6, 4, 3, 2};
+    static const int radices[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
     unsigned long long p = 0;
     int rem = M, st = 0;
     while (rem > 1) {
         int pick = 0;
-        for (int R : radices)
-            if (ept % R == 0 && rem % R == 0) { pick = R; break; }
+        for (int R : radices) {
+            if (R > ept || rem % R) continue;
+            const int nb = (M / R + tpl - 1) / tpl;
+            if (nb > ept / R) continue;
+            pick = R;
+            break;
+        }
         if (!pick || st >= 8) return false;
         p |= (unsigned long long)pick << (8 * st++);
         rem /= pick;
@@ -113,19 +120,20 @@ bool Context::config(int N, int prec, bool rows, AxisCfg *out)
         return true;
     }
     const int M = N / 2;
-    static const int epts[] = {16, 12, 8, 6, 4, 3, 2, 1};
+    static const int epts[] = {16, 12, 10, 8, 6, 5, 4};
     AxisCfg c;
     c.N = N;
     c.M = M;
     for (int e : epts) {
-        if (M % e) continue;
-        if (M / e > 256) continue;
-        unsigned long long p;
-        if (!radix_plan(M, e, &p)) continue;
-        c.EPT = e;
-        c.TPL = M / e;
-        c.plan = p;
-        break;
+        if (c.EPT) break;
+        for (int tpl = (M + e - 1) / e; tpl <= 256 && tpl <= (M > 1 ? M : 1); tpl++) {
+            unsigned long long p;
+            if (!radix_plan(M, e, tpl, &p)) continue;
+            c.EPT = e;
+            c.TPL = tpl;
+            c.plan = p;
+            break;
+        }
     }
     if (!c.EPT) return false;
     int L = 256 / c.TPL;
@@ -153,8 +161,56 @@ void *Context::workspace(size_t bytes, int slot)
     return d;
 }
 
+void *Context::cached_blob(const std::string &key, const void *data, size_t bytes)
+{
+    auto it = blobs_.find(key);
+    if (it != blobs_.end()) return it->second;
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, data, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    blobs_[key] = d;
+    return d;
+}
+
+void Context::timing_mark(const char *name, hipStream_t s)
+{
+    if ((int)ev_.size() <= nev_) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ev_.push_back(e);
+    }
+    if (hipEventRecord(ev_[nev_], s) != hipSuccess) return;
+    nev_++;
+    names_.push_back(name);
+}
+
+void Context::timing_end(hipStream_t s) { timing_mark("", s); }
+
+int Context::timing_read(double *ms, int max, std::string *names)
+{
+    if (nev_ == 0) return 0;
+    if (hipEventSynchronize(ev_[nev_ - 1]) != hipSuccess) return -1;
+    int k = 0;
+    for (int i = 0; i + 1 < nev_ && k < max; i++) {
+        if (names_[i].empty()) continue;           // end-of-call marker
+        float t = 0;
+        if (hipEventElapsedTime(&t, ev_[i], ev_[i + 1]) != hipSuccess) return -1;
+        ms[k] = t;
+        if (names) names[k] = names_[i];
+        k++;
+    }
+    nev_ = 0;
+    names_.clear();
+    return k;
+}
+
 void Context::release()
 {
+    for (auto &kv : blobs_) (void)hipFree(kv.second);
+    blobs_.clear();
     for (auto &kv : tw_) (void)hipFree(kv.second);
     tw_.clear();
     for (auto &kv : ws_) (void)hipFree(kv.second.first);

@@ -37,7 +37,18 @@ public:
     void release();
     // persistent workspace (grown on demand, kept until release())
     void *workspace(size_t bytes, int slot = 0);
+    // device copy of a small host table, cached by its bytes (kept until release())
+    void *cached_blob(const std::string &key, const void *data, size_t bytes);
     std::mutex &mutex() { return mu_; }
+    // per-pass timing (the counterpart of the reference's TIME_TEST splits):
+    // when enabled, every pass of a transform / solve records an event at its
+    // start and each call one at its end; events accumulate until read.
+    void set_timing(bool on) { timing_ = on; nev_ = 0; names_.clear(); }
+    bool timing() const { return timing_; }
+    void timing_mark(const char *name, hipStream_t s);   // start of a pass named `name`
+    void timing_end(hipStream_t s);                      // end of the call's last pass
+    // durations of the recorded passes (waits for the events), then clears
+    int timing_read(double *ms, int max, std::string *names);
 
 private:
     explicit Context(int dev);
@@ -47,9 +58,23 @@ private:
     std::map<std::pair<int, int>, void *> tw_;
     std::map<std::tuple<int, int, bool>, AxisCfg> cfg_;
     std::map<int, std::pair<void *, size_t>> ws_;
+    std::map<std::string, void *> blobs_;
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_;
+    int nev_ = 0;
+    std::vector<std::string> names_;
 };
 
-// radix plan of M with every radix dividing ept; false if none
-bool radix_plan(int M, int ept, unsigned long long *plan);
+// records pass boundaries while the context's timing is on
+struct PassTimer {
+    Context &ctx;
+    hipStream_t s;
+    PassTimer(Context &c, hipStream_t st) : ctx(c), s(st) {}
+    void mark(const char *name) { if (ctx.timing()) ctx.timing_mark(name, s); }
+    ~PassTimer() { if (ctx.timing()) ctx.timing_end(s); }
+};
+
+// radix plan of an M-point FFT for tpl threads of at most ept points each
+bool radix_plan(int M, int ept, int tpl, unsigned long long *plan);
 
 }  // namespace chz

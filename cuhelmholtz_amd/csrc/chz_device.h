@@ -12,9 +12,9 @@
 // touch consecutive lines: every LDS access of the fold, FFT and unfold
 // phases is a contiguous run of L points.
 //
-// The FFT is a Stockham autosort over M points with each thread holding EPT
-// points in registers (M = TPL * EPT); a stage of radix R runs EPT/R
-// butterflies per thread.  Only the forward transform (e^{-2 pi i/M}) is
+// The FFT is a Stockham autosort over M points with each thread holding up to
+// EPT points in registers (M <= TPL * EPT); a stage of radix R runs up to
+// EPT/R butterflies per thread.  Only the forward transform (e^{-2 pi i/M}) is
 // implemented; the inverse used by the C2R-type kinds is conj(FFT(conj(.))),
 // folded into the fold and unfold phases.
 #pragma once
@@ -54,11 +54,28 @@ __host__ __device__ constexpr double cos48(int k)
              : 0.0;                               // k == 12
     return neg ? -v : v;
 }
-// W_R^e = e^{-2 pi i e / R}: (cos, -sin)
+// cos(2 pi k / 20) for any integer k (multiples of 18 degrees, radices 5, 10, 20)
+__host__ __device__ constexpr double cos20(int k)
+{
+    k %= 20;
+    if (k < 0) k += 20;
+    if (k > 10) k = 20 - k;
+    bool neg = false;
+    if (k > 5) { k = 10 - k; neg = true; }
+    double v = k == 0 ? 1.0
+             : k == 1 ? 0.95105651629515357212    // cos(pi/10)
+             : k == 2 ? 0.80901699437494742410    // cos(pi/5)
+             : k == 3 ? 0.58778525229247312917    // cos(3pi/10)
+             : k == 4 ? 0.30901699437494742410    // cos(2pi/5)
+             : 0.0;                               // k == 5
+    return neg ? -v : v;
+}
+// W_R^e = e^{-2 pi i e / R}: (cos, -sin); -sin(a) = -cos(pi/2 - a)
 template <int R, int E> struct Root {
-    static constexpr int k = ((E % R) * (48 / R));
-    static constexpr double c = cos48(k);
-    static constexpr double s = -cos48(12 - k);   // -sin(a) = -cos(pi/2 - a)
+    static constexpr bool b48 = 48 % R == 0;
+    static constexpr int k = (E % R) * (b48 ? 48 / R : 20 / R);
+    static constexpr double c = b48 ? cos48(k) : cos20(k);
+    static constexpr double s = b48 ? -cos48(12 - k) : -cos20(5 - k);
 };
 
 template <int Begin, int End, typename F>
@@ -121,6 +138,25 @@ template <> struct Dft<4> {
         x[o + 3 * st] = csub(t1, t3);
     }
 };
+template <> struct Dft<5> {
+    template <typename V> __device__ __forceinline__ static void run(V *x, int o, int st)
+    {
+        typedef decltype(V().x) T;
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;   // cos(2pi/5), cos(4pi/5)
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;    // sin(2pi/5), sin(4pi/5)
+        V a0 = x[o], a1 = x[o + st], a2 = x[o + 2 * st], a3 = x[o + 3 * st], a4 = x[o + 4 * st];
+        V t1 = cadd(a1, a4), t2 = cadd(a2, a3), t3 = csub(a1, a4), t4 = csub(a2, a3);
+        V b1 = mk<V>(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+        V b2 = mk<V>(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+        V d1 = mk<V>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        V d2 = mk<V>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        x[o] = mk<V>(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+        x[o + st] = cadd(b1, cmul_mi(d1));
+        x[o + 4 * st] = csub(b1, cmul_mi(d1));
+        x[o + 2 * st] = cadd(b2, cmul_mi(d2));
+        x[o + 3 * st] = csub(b2, cmul_mi(d2));
+    }
+};
 // R = P*Q: input index n = Q*n1 + n2, output index k = k1 + P*k2.
 // P-point DFTs over n1 for each n2, twiddle W_R^{n2 k1}, Q-point DFTs over n2.
 template <int P, int Q> struct DftPQ {
@@ -148,6 +184,7 @@ template <int P, int Q> struct DftPQ {
     }
 };
 template <> struct Dft<6> : DftPQ<2, 3> {};
+template <> struct Dft<10> : DftPQ<2, 5> {};
 template <> struct Dft<8> : DftPQ<2, 4> {};
 template <> struct Dft<12> : DftPQ<4, 3> {};
 template <> struct Dft<16> : DftPQ<4, 4> {};
@@ -162,34 +199,42 @@ template <typename T> struct Tile {
     __device__ __forceinline__ T &re(int s) const { return reinterpret_cast<T *>(buf)[(size_t)((s >> 1) * LS + c) * 2 + (s & 1)]; }
 };
 
-// One Stockham stage of radix R over M = TPL*EPT points, in place.
-// Reads points u + TPL*e (e = 0..EPT-1), writes the permuted outputs.
+// One Stockham stage of radix R over M points, in place.  Thread u of a line
+// runs the butterflies b = u + i*TPL (i < EPT/R, b < M/R), so the stage fits
+// when ceil(M/(R*TPL)) <= EPT/R (the host planner guarantees it).
 template <int R, int EPT, typename T>
 __device__ __forceinline__ void fft_stage(const Tile<T> &tl, int u, int TPL, int M, int Ns,
                                           const typename vec2_of<T>::type *__restrict__ tw)
 {
     typedef typename vec2_of<T>::type V;
     constexpr int NB = EPT / R;
-    V v[EPT];
+    const int MR = M / R;
+    V v[NB * R];
 #pragma unroll
-    for (int e = 0; e < EPT; e++) v[e] = tl.pt(u + TPL * e);
+    for (int i = 0; i < NB; i++) {
+        const int b = u + i * TPL;
+        if (b < MR) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i * R + r] = tl.pt(b + r * MR);
+        }
+    }
     __syncthreads();
     const int twstride = M / (Ns * R);
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int b = u + i * TPL;
-        const int j = b % Ns;
-        V x[R];
+        if (b < MR) {
+            const int j = b % Ns;
+            V *x = v + i * R;
+            if (Ns > 1) {
 #pragma unroll
-        for (int r = 0; r < R; r++) x[r] = v[i + r * NB];
-        if (Ns > 1) {
+                for (int r = 1; r < R; r++) x[r] = cmul(x[r], tw[r * j * twstride]);
+            }
+            Dft<R>::run(x, 0, 1);
+            const int base = (b - j) * R + j;
 #pragma unroll
-            for (int r = 1; r < R; r++) x[r] = cmul(x[r], tw[r * j * twstride]);
+            for (int q = 0; q < R; q++) tl.pt(base + q * Ns) = x[q];
         }
-        Dft<R>::run(x, 0, 1);
-        const int base = (b - j) * R + j;
-#pragma unroll
-        for (int q = 0; q < R; q++) tl.pt(base + q * Ns) = x[q];
     }
     __syncthreads();
 }
@@ -205,13 +250,15 @@ __device__ __forceinline__ void fft_forward(const Tile<T> &tl, int u, int TPL, i
         const int R = (int)((plan >> (8 * st)) & 255);
         if (R == 0) break;
         switch (R) {
-            case 2: if constexpr (EPT % 2 == 0) fft_stage<2, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 3: if constexpr (EPT % 3 == 0) fft_stage<3, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 4: if constexpr (EPT % 4 == 0) fft_stage<4, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 6: if constexpr (EPT % 6 == 0) fft_stage<6, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 8: if constexpr (EPT % 8 == 0) fft_stage<8, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 12: if constexpr (EPT % 12 == 0) fft_stage<12, EPT>(tl, u, TPL, M, Ns, tw); break;
-            case 16: if constexpr (EPT % 16 == 0) fft_stage<16, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 2: if constexpr (EPT >= 2) fft_stage<2, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 3: if constexpr (EPT >= 3) fft_stage<3, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 4: if constexpr (EPT >= 4) fft_stage<4, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 5: if constexpr (EPT >= 5) fft_stage<5, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 6: if constexpr (EPT >= 6) fft_stage<6, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 8: if constexpr (EPT >= 8) fft_stage<8, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 10: if constexpr (EPT >= 10) fft_stage<10, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 12: if constexpr (EPT >= 12) fft_stage<12, EPT>(tl, u, TPL, M, Ns, tw); break;
+            case 16: if constexpr (EPT >= 16) fft_stage<16, EPT>(tl, u, TPL, M, Ns, tw); break;
             default: break;
         }
         Ns *= R;

@@ -1,0 +1,7 @@
+// FP32 axis pass, column tiles (y, z): instantiations for every elements-per-thread value.
+#include "chz_axis_kernels.h"
+
+namespace chz {
+#define KSEL(e) (const void *)k_axis_pass<float, float, float, e, false>
+const void *axis_kernel_f32_cols(int ept) { CHZ_EPT_SWITCH(ept, KSEL) }
+}  // namespace chz

@@ -39,12 +39,37 @@ struct PassArgs {
     double scale;
 };
 
+// Helmholtz-solve extras of a pass: boundary data for the folds of pass 1,
+// eigenvalue tables and the inverse kind of the fused middle z pass.
+struct SolveArgs {
+    const double *f;                                   // node array (NX+1)(NY+1)(NZ+1)
+    const double *bcl, *bcr, *bcb, *bct, *bcf, *bce;   // Neumann face data (may be null)
+    int NX, NY, NZ;
+    int bc[3], s[3], e[3];                             // codes and unknown windows per axis
+    double ih[3], ih2[3];                              // 1/h, 1/h^2
+    const double *eig;                                 // slot eigenvalue tables x | y | z
+    double lambda;
+    int singular;                                      // zero the (0,0,0) mode
+    int kind2, out2_lo, out2_hi;                       // inverse kind of the middle pass and its window
+};
+
 __host__ __device__ inline int lds_points(int M, int L) { return (M + 1) * (L + 1); }
+
+// kernel getters (one translation unit per precision)
+const void *axis_kernel_f64_rows(int ept);
+const void *axis_kernel_f64_cols(int ept);
+const void *axis_kernel_f32_rows(int ept);
+const void *axis_kernel_f32_cols(int ept);
+const void *solve_xfwd_kernel_f64(int ept);
+const void *solve_xfwd_kernel_f32(int ept);
+const void *solve_zmid_kernel_f64(int ept);
+const void *solve_zmid_kernel_f32(int ept);
+const void *periodic_copy_kernel();
 
 // Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
 // FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.
 hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s);
-const void *axis_pass_kernel(int prec, int ept, bool rows);
+hipError_t launch_solve_pass(const void *fn, const PassArgs &p, const SolveArgs &sa, int prec, int nb, hipStream_t s);
 size_t axis_pass_lds(int prec, int M, int L, int TPL);
 
 }  // namespace chz

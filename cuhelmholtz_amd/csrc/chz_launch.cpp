@@ -1,0 +1,33 @@
+// chz_launch.cpp -- host-side launch of the axis and solver passes.
+#include "chz_kernels.h"
+
+namespace chz {
+
+size_t axis_pass_lds(int prec, int M, int L, int TPL)
+{
+    const size_t esz = prec == 64 ? 8 : 4;
+    return (size_t)lds_points(M, L) * 2 * esz + (size_t)TPL * L * esz;
+}
+
+hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s)
+{
+    const void *fn = prec == 64 ? (rows ? axis_kernel_f64_rows(ept) : axis_kernel_f64_cols(ept))
+                                : (rows ? axis_kernel_f32_rows(ept) : axis_kernel_f32_cols(ept));
+    if (!fn) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((p.na + p.L - 1) / p.L), (unsigned)nb, 1);
+    PassArgs a = p;
+    void *args[] = {&a};
+    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
+}
+
+hipError_t launch_solve_pass(const void *fn, const PassArgs &p, const SolveArgs &sa, int prec, int nb, hipStream_t s)
+{
+    if (!fn) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((p.na + p.L - 1) / p.L), (unsigned)nb, 1);
+    PassArgs a = p;
+    SolveArgs b = sa;
+    void *args[] = {&a, &b};
+    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
+}
+
+}  // namespace chz

@@ -1,0 +1,308 @@
+// chz_solver.cpp -- FFT-diagonalised 7-point Helmholtz solve (plain FP64).
+//
+// The reference (cudahelmholtz/cuda_helmholtz_solver.cu:33-380) folds the
+// boundary data into f, gathers the unknown box into a padded buffer it
+// allocates per call, runs a forward do_transform (12 sweeps with its
+// transposes), an eigenvalue divide kernel, an inverse do_transform, a store
+// kernel and the periodic copies, synchronising after most steps.  Here the
+// same arithmetic runs as 5 stream-ordered passes over a persistent workspace:
+//   1  rows of f:   folds + gather + forward x  -> ex
+//   2  y columns:   forward y                    (in place)
+//   3  z columns:   forward z, divide, inverse z (in place, one pass)
+//   4  y columns:   inverse y                    (in place)
+//   5  rows of ex:  inverse x, x ratio          -> f (unknown box)
+// plus face kernels for the periodic duplicate nodes: 80 bytes per cell.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chz_context.h"
+#include "chz_solver.h"
+#include "chz_transform.h"
+
+namespace chz {
+
+// bc code -> unknown window and transform kinds (cuda_helmholtz_solver.cu:78-172)
+static bool bc_kinds(int bc, int N, int *s, int *e, int *fwd, int *inv)
+{
+    switch (bc) {
+        case 0: *s = 0; *e = N - 1; *fwd = K_R2C; *inv = K_C2R; return true;
+        case 1: *s = 1; *e = N - 1; *fwd = K_DST_1; *inv = K_DST_1; return true;
+        case 2: *s = 1; *e = N; *fwd = K_DST_3; *inv = K_DST_2; return true;
+        case 3: *s = 0; *e = N; *fwd = K_DCT_1; *inv = K_DCT_1; return true;
+        case 4: *s = 0; *e = N - 1; *fwd = K_DCT_3; *inv = K_DCT_2; return true;
+    }
+    return false;
+}
+
+int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3], double lambda,
+              const double *const faces[6])
+{
+    static const char *ax = "xyz";
+    for (int a = 0; a < 3; a++) {
+        g.N[a] = N[a];
+        g.bc[a] = bc[a];
+        if (!bc_kinds(bc[a], N[a], &g.s[a], &g.e[a], &g.fwd[a], &g.inv[a]))
+            return fail(ERR_INVALID, std::string("bad boundary condition along ") + ax[a]);
+        if (N[a] < 2 || (N[a] & 1)) return fail(ERR_INVALID, std::string("cell count must be even and >= 2 along ") + ax[a]);
+        const double len = dom[2 * a + 1] - dom[2 * a];
+        if (!(len > 0)) return fail(ERR_INVALID, std::string("empty domain along ") + ax[a]);
+        g.ih[a] = N[a] / len;                         // cuda_helmholtz_solver.cu:44-46
+        g.ih2[a] = g.ih[a] * g.ih[a];
+        const bool need_lo = bc[a] == 3 || bc[a] == 4, need_hi = bc[a] == 2 || bc[a] == 3;
+        if ((need_lo && !faces[2 * a]) || (need_hi && !faces[2 * a + 1]))
+            return fail(ERR_INVALID, std::string("missing Neumann boundary array along ") + ax[a]);
+        g.faces[2 * a] = need_lo ? faces[2 * a] : nullptr;
+        g.faces[2 * a + 1] = need_hi ? faces[2 * a + 1] : nullptr;
+    }
+    g.lambda = lambda;
+    g.singular = lambda == 0.0 && (bc[0] == 0 || bc[0] == 3) && (bc[1] == 0 || bc[1] == 3) && (bc[2] == 0 || bc[2] == 3);
+    // ratio = 1 / prod(N_a/2 * (2 if periodic)), cuda_helmholtz_solver.cu:211-215
+    double r = 1.0;
+    for (int a = 0; a < 3; a++) r *= (double)(N[a] / 2) * (bc[a] == 0 ? 2.0 : 1.0);
+    g.ratio = 1.0 / r;
+    return OK;
+}
+
+// Slot eigenvalue tables (2 cos(m theta + shift) - 2)/h^2 of the three axes,
+// host FP64 (cuda_helmholtz_solver.cu:58-73, _cuda_helmholtz_.inl:413-418).
+static std::vector<double> eig_tables(const SolveGeom &g)
+{
+    const double pi = 3.14159265358979323846;
+    std::vector<double> t;
+    for (int a = 0; a < 3; a++) {
+        const int N = g.N[a];
+        double theta = pi / N, shift = 0;
+        if (g.bc[a] == 0) theta *= 2;
+        if (g.bc[a] == 2) shift = -0.5 * pi / N;
+        if (g.bc[a] == 4) shift = 0.5 * pi / N;
+        for (int sl = 0; sl < N + 2; sl++) {
+            const int m = g.bc[a] == 0 ? sl / 2 : sl;
+            t.push_back((2 * std::cos(m * theta + shift) - 2) * g.ih2[a]);
+        }
+    }
+    return t;
+}
+
+const double *solver_eig(Context &ctx, const SolveGeom &g)
+{
+    std::vector<double> t = eig_tables(g);
+    std::string key(reinterpret_cast<const char *>(t.data()), t.size() * sizeof(double));
+    void *d = ctx.cached_blob(key, t.data(), t.size() * sizeof(double));
+    return reinterpret_cast<const double *>(d);
+}
+
+void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const double *eig)
+{
+    memset(&sa, 0, sizeof(sa));
+    sa.f = f;
+    sa.bcl = g.faces[0]; sa.bcr = g.faces[1];
+    sa.bcb = g.faces[2]; sa.bct = g.faces[3];
+    sa.bcf = g.faces[4]; sa.bce = g.faces[5];
+    sa.NX = g.N[0]; sa.NY = g.N[1]; sa.NZ = g.N[2];
+    for (int a = 0; a < 3; a++) {
+        sa.bc[a] = g.bc[a];
+        sa.s[a] = g.s[a];
+        sa.e[a] = g.e[a];
+        sa.ih[a] = g.ih[a];
+        sa.ih2[a] = g.ih2[a];
+    }
+    sa.eig = eig;
+    sa.lambda = g.lambda;
+    sa.singular = g.singular ? 1 : 0;
+}
+
+// The 5-pass spectral core on workspace ex (padded, precision of T) with the
+// rhs read from f (node layout, FP64) and the solution written back to f.
+template <typename T>
+int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st)
+{
+    const int prec = sizeof(T) == 8 ? 64 : 32;
+    const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
+    AxisCfg cx, cy, cz;
+    if (!ctx.config(NX, prec, true, &cx) || !ctx.config(NY, prec, false, &cy) || !ctx.config(NZ, prec, false, &cz))
+        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
+    const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
+    const double *eig = solver_eig(ctx, g);
+    if (!twx || !twy || !twz || !eig) return fail(ERR_ALLOC, "table allocation failed");
+
+    const long long fx = NX + 1, fy = NY + 1, P0 = NX + 2, P1 = NY + 2;
+    const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
+    int sxl, sxh, syl, syh, szl, szh;   // spectral windows (forward outputs)
+    out_window(g.fwd[0], NX, &sxl, &sxh);
+    out_window(g.fwd[1], NY, &syl, &syh);
+    out_window(g.fwd[2], NZ, &szl, &szh);
+    SolveArgs sa;
+    fill_solve_args(sa, g, f, eig);
+    int rc;
+    hipError_t e;
+    const bool tm = ctx.timing();
+
+    // pass 1: folds + forward x, f rows (j,k) of the unknown box -> ex
+    {
+        PassArgs p{};
+        fill_pass(p, cx, prec, g.fwd[0], twx);
+        p.src = f; p.dst = ex;
+        p.s_sa = fx; p.s_sb = fx * fy; p.s_sp = 1; p.s_rowlen = (int)fx;
+        p.d_sa = P0; p.d_sb = P0 * P1; p.d_sp = 1; p.d_rowlen = (int)P0;
+        p.a0 = g.s[1]; p.na = g.e[1] - g.s[1] + 1; p.b0 = g.s[2];
+        p.in_lo = g.s[0]; p.in_hi = g.e[0];
+        p.st_lo = sxl; p.st_hi = sxh;
+        p.out_lo = sxl; p.out_hi = sxh;
+        const int nb = g.e[2] - g.s[2] + 1;
+        if (!pass_in_bounds(p, true, nb, fvol, vol)) return fail(ERR_INVALID, "pass 1 geometry outside the buffers");
+        const void *fn = prec == 64 ? solve_xfwd_kernel_f64(cx.EPT) : solve_xfwd_kernel_f32(cx.EPT);
+        if (tm) ctx.timing_mark("xfwd", st);
+        if ((e = launch_solve_pass(fn, p, sa, prec, nb, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("pass 1 launch: ") + hipGetErrorString(e));
+    }
+    // pass 2: forward y, columns x in the x spectral window, z in the unknown window
+    {
+        PassArgs p{};
+        fill_pass(p, cy, prec, g.fwd[1], twy);
+        p.src = ex; p.dst = ex;
+        p.s_sa = 1; p.s_sb = P0 * P1; p.s_sp = P0;
+        p.d_sa = 1; p.d_sb = P0 * P1; p.d_sp = P0;
+        p.a0 = sxl; p.na = sxh - sxl + 1; p.b0 = g.s[2];
+        p.in_lo = g.s[1]; p.in_hi = g.e[1];
+        p.st_lo = syl; p.st_hi = syh;
+        p.out_lo = syl; p.out_hi = syh;
+        if (tm) ctx.timing_mark("yfwd", st);
+        if ((rc = launch_checked(p, prec, cy.EPT, false, g.e[2] - g.s[2] + 1, vol, vol, st, "pass 2"))) return rc;
+    }
+    // pass 3: forward z, divide, inverse z
+    {
+        PassArgs p{};
+        fill_pass(p, cz, prec, g.fwd[2], twz);
+        p.src = ex; p.dst = ex;
+        p.s_sa = 1; p.s_sb = P0; p.s_sp = P0 * P1;
+        p.d_sa = 1; p.d_sb = P0; p.d_sp = P0 * P1;
+        p.a0 = sxl; p.na = sxh - sxl + 1; p.b0 = syl;
+        p.in_lo = g.s[2]; p.in_hi = g.e[2];
+        p.st_lo = g.s[2]; p.st_hi = g.e[2];
+        p.out_lo = szl; p.out_hi = szh;
+        SolveArgs s3 = sa;
+        s3.kind2 = g.inv[2];
+        s3.out2_lo = g.s[2];
+        s3.out2_hi = g.e[2];
+        const int nb = syh - syl + 1;
+        if (!pass_in_bounds(p, false, nb, vol, vol)) return fail(ERR_INVALID, "pass 3 geometry outside the buffers");
+        const void *fn = prec == 64 ? solve_zmid_kernel_f64(cz.EPT) : solve_zmid_kernel_f32(cz.EPT);
+        if (tm) ctx.timing_mark("zmid", st);
+        if ((e = launch_solve_pass(fn, p, s3, prec, nb, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("pass 3 launch: ") + hipGetErrorString(e));
+    }
+    // pass 4: inverse y
+    {
+        PassArgs p{};
+        fill_pass(p, cy, prec, g.inv[1], twy);
+        p.src = ex; p.dst = ex;
+        p.s_sa = 1; p.s_sb = P0 * P1; p.s_sp = P0;
+        p.d_sa = 1; p.d_sb = P0 * P1; p.d_sp = P0;
+        p.a0 = sxl; p.na = sxh - sxl + 1; p.b0 = g.s[2];
+        p.in_lo = syl; p.in_hi = syh;
+        p.st_lo = g.s[1]; p.st_hi = g.e[1];
+        p.out_lo = g.s[1]; p.out_hi = g.e[1];
+        if (tm) ctx.timing_mark("yinv", st);
+        if ((rc = launch_checked(p, prec, cy.EPT, false, g.e[2] - g.s[2] + 1, vol, vol, st, "pass 4"))) return rc;
+    }
+    return OK;
+}
+
+// pass 5 of the plain solve: inverse x of the ex rows, x ratio, store into f
+template <typename T>
+int solve_store(Context &ctx, const SolveGeom &g, double *f, const T *ex, hipStream_t st)
+{
+    const int prec = sizeof(T) == 8 ? 64 : 32;
+    const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
+    AxisCfg cx;
+    if (!ctx.config(NX, prec, true, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
+    const void *twx = ctx.twiddles(NX, prec);
+    const long long fx = NX + 1, fy = NY + 1, P0 = NX + 2, P1 = NY + 2;
+    const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
+    int sxl, sxh;
+    out_window(g.fwd[0], NX, &sxl, &sxh);
+    PassArgs p{};
+    fill_pass(p, cx, prec, g.inv[0], twx);
+    p.src = ex; p.dst = f;
+    p.s_sa = P0; p.s_sb = P0 * P1; p.s_sp = 1; p.s_rowlen = (int)P0;
+    p.d_sa = fx; p.d_sb = fx * fy; p.d_sp = 1; p.d_rowlen = (int)fx;
+    p.a0 = g.s[1]; p.na = g.e[1] - g.s[1] + 1; p.b0 = g.s[2];
+    p.in_lo = sxl; p.in_hi = sxh;
+    p.st_lo = g.s[0]; p.st_hi = g.e[0];
+    p.out_lo = g.s[0]; p.out_hi = g.e[0];
+    p.scale = g.ratio;
+    static_assert(sizeof(T) == 8, "the FP64 store pass writes the solution directly");
+    if (ctx.timing()) ctx.timing_mark("xinv", st);
+    return launch_checked(p, prec, cx.EPT, true, g.e[2] - g.s[2] + 1, vol, fvol, st, "pass 5");
+}
+
+int periodic_copies(const SolveGeom &g, double *f, hipStream_t st)
+{
+    const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
+    for (int a = 0; a < 3; a++) {
+        if (g.bc[a] != 0) continue;
+        const long long n = a == 0 ? (long long)(NY + 1) * (NZ + 1) : a == 1 ? (long long)(NX + 1) * (NZ + 1)
+                                                                             : (long long)(NX + 1) * (NY + 1);
+        int nx = NX, ny = NY, nz = NZ, axis = a;
+        void *args[] = {&f, &nx, &ny, &nz, &axis};
+        hipError_t e = hipLaunchKernel(periodic_copy_kernel(), dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0, st);
+        if (e != hipSuccess) return fail(ERR_DEVICE, std::string("periodic copy launch: ") + hipGetErrorString(e));
+    }
+    return OK;
+}
+
+int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
+                   double *f, hipStream_t st)
+{
+    if (!f) return fail(ERR_INVALID, "null solution array");
+    SolveGeom g;
+    int rc = make_geom(g, dom, N, bc, lambda, faces);
+    if (rc) return rc;
+    Context &ctx = Context::get();
+    std::lock_guard<std::mutex> lock(ctx.mutex());
+    const size_t bytes = sizeof(double) * (size_t)(N[0] + 2) * (N[1] + 2) * (N[2] + 2);
+    double *ex = reinterpret_cast<double *>(ctx.workspace(bytes, 0));
+    if (!ex) return fail(ERR_ALLOC, "workspace allocation failed");
+    PassTimer timer(ctx, st);
+    if ((rc = solve_core<double>(ctx, g, f, ex, st))) return rc;
+    if ((rc = solve_store<double>(ctx, g, f, ex, st))) return rc;
+    timer.mark("periodic");
+    return periodic_copies(g, f, st);
+}
+
+template int solve_core<float>(Context &, const SolveGeom &, double *, float *, hipStream_t);
+
+}  // namespace chz
+
+extern "C" {
+
+int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                        double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                        double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                        double lambda, double *f, void *stream)
+{
+    const double dom[6] = {left, right, bottom, top, front, back};
+    const int N[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
+    const double *faces[6] = {bcl, bcr, bcb, bct, bcf, bce};
+    return chz::helmholtz_fp64(dom, N, bc, faces, lambda, f, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// C++ drop-in: cudahelmholtz/cuda_helmholtz_solver.h:5-11 (synchronous like the reference)
+int cuda_helmholtz_solver(double left, double right, int NX, int xbc, double *bcl, double *bcr, double bottom,
+                          double top, int NY, int ybc, double *bcb, double *bct, double front, double back, int NZ,
+                          int zbc, double *bcf, double *bce, double lambda, double *f)
+{
+    int rc = chz_helmholtz_solve(left, right, NX, xbc, bcl, bcr, bottom, top, NY, ybc, bcb, bct, front, back, NZ, zbc,
+                                 bcf, bce, lambda, f, nullptr);
+    if (rc == 0) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = chz::fail(chz::ERR_DEVICE, std::string("device error: ") + hipGetErrorString(e));
+    }
+    if (rc != 0) printf("cuhelmholtz_amd: %s\n", chz::last_error());
+    return rc;
+}

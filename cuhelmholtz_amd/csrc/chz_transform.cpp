@@ -8,6 +8,7 @@
 // with no transposes, all stream-ordered.
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "chz_context.h"
 #include "chz_transform.h"
@@ -68,6 +69,7 @@ bool pass_in_bounds(const PassArgs &p, bool rows, int nb, long long src_elems, l
     long long smax, dmax;
     if (rows) {
         if (p.s_sa != p.s_rowlen || p.d_sa != p.d_rowlen || p.s_sp != 1 || p.d_sp != 1) return false;
+        if (p.st_hi >= p.d_rowlen) return false;
         smax = a1 * p.s_sa + b1 * p.s_sb + p.s_rowlen - 1;
         dmax = a1 * p.d_sa + b1 * p.d_sb + p.d_rowlen - 1;
     } else {
@@ -98,7 +100,7 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
     std::lock_guard<std::mutex> g(ctx.mutex());
     AxisCfg cx, cy, cz;
     if (!ctx.config(NX, prec, true, &cx) || !ctx.config(NY, prec, false, &cy) || !ctx.config(NZ, prec, false, &cz))
-        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s and 3s, N <= 4096)");
+        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
     const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
     if (!twx || !twy || !twz) return fail(ERR_ALLOC, "twiddle table allocation failed");
 
@@ -112,6 +114,7 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
     out_window(ky, NY, &oyl, &oyh);
     out_window(kz, NZ, &ozl, &ozh);
     const long long vol = P0 * P1 * (long long)(NZ + 2);
+    PassTimer timer(ctx, s);
 
     // X pass: rows (y, z), y in in-window(Y), z in in-window(Z); src -> dst
     {
@@ -126,6 +129,7 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
         p.in_lo = ixl; p.in_hi = ixh;
         p.st_lo = 0; p.st_hi = NX + 1;
         p.out_lo = oxl; p.out_hi = oxh;
+        timer.mark("x");
         if ((rc = launch_checked(p, prec, cx.EPT, true, izh - izl + 1, vol, vol, s, "x pass"))) return rc;
     }
     // Y pass: columns x in out-window(X), z in in-window(Z); in place on dst
@@ -140,6 +144,7 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
         p.in_lo = iyl; p.in_hi = iyh;
         p.st_lo = 0; p.st_hi = NY + 1;
         p.out_lo = oyl; p.out_hi = oyh;
+        timer.mark("y");
         if ((rc = launch_checked(p, prec, cy.EPT, false, izh - izl + 1, vol, vol, s, "y pass"))) return rc;
     }
     // Z pass: columns x in out-window(X), y in out-window(Y); in place on dst
@@ -154,6 +159,7 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
         p.in_lo = izl; p.in_hi = izh;
         p.st_lo = 0; p.st_hi = NZ + 1;
         p.out_lo = ozl; p.out_hi = ozh;
+        timer.mark("z");
         if ((rc = launch_checked(p, prec, cz.EPT, false, oyh - oyl + 1, vol, vol, s, "z pass"))) return rc;
     }
     return OK;
@@ -201,5 +207,22 @@ int chz_axis_config(int N, int prec, int rows, int *ept, int *tpl, int *lines, u
 }
 
 int chz_version(void) { return 100; }
+
+void chz_set_timing(int on) { chz::Context::get().set_timing(on != 0); }
+
+int chz_get_timing(double *ms, int max_passes, char *names, int names_len)
+{
+    chz::Context &ctx = chz::Context::get();
+    const int cap = max_passes < 4096 ? max_passes : 4096;
+    std::vector<std::string> nm((size_t)(cap > 0 ? cap : 1));
+    int n = ctx.timing_read(ms, cap, nm.data());
+    if (n < 0) return chz::fail(chz::ERR_DEVICE, "timing events unavailable");
+    if (names && names_len > 0) {
+        std::string all;
+        for (int i = 0; i < n; i++) all += (i ? "," : "") + nm[i];
+        snprintf(names, (size_t)names_len, "%s", all.c_str());
+    }
+    return n;
+}
 
 }  // extern "C"

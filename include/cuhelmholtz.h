@@ -9,7 +9,7 @@
  * -3 = device / launch error, -4 = device allocation failure.
  * chz_last_error() returns the message of the last failure on this thread.
  *
- * Supported sizes: even N with N/2 = 2^a 3^b, N <= 4096 per axis.
+ * Supported sizes: even N with N/2 = 2^a 3^b 5^c (up to 4096 per axis).
  */
 #ifndef CUHELMHOLTZ_H
 #define CUHELMHOLTZ_H
@@ -42,6 +42,21 @@ int chz_do_transform(double *d_data, double *d_out, int NX, int NY, int NZ,
 int chz_do_transform_f32(float *d_data, float *d_out, int NX, int NY, int NZ,
                          int x_kind, int y_kind, int z_kind, void *stream);
 
+/* FFT-diagonalised solve of laplace u + lambda u = f (7-point stencil) on the
+ * node grid of [left,right]x[bottom,top]x[front,back] with NX, NY, NZ cells,
+ * FP64 throughout.  Same arguments, layouts and boundary codes as
+ * cuda_helmholtz_solver (reference cudahelmholtz/cuda_helmholtz_solver.h:5-11,
+ * .cu:33-380): f holds (NX+1)(NY+1)(NZ+1) nodes (x fastest) with the rhs and
+ * the Dirichlet values on entry and the solution on return; codes 0 periodic,
+ * 1 D-D, 2 D-N, 3 N-N, 4 N-D; Neumann face data (derivative along +axis) x:
+ * [k*(NY+1)+j], y: [k*(NX+1)+i], z: [j*(NX+1)+i], NULL where unused.  For
+ * lambda == 0 with only periodic / Neumann axes the (0,0,0) mode is set to 0.
+ * Uses a persistent per-device workspace of (NX+2)(NY+2)(NZ+2) doubles. */
+int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                        double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                        double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                        double lambda, double *f, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *chz_last_error(void);
 
@@ -56,6 +71,16 @@ int chz_axis_config(int N, int prec, int rows, int *ept, int *tpl, int *lines,
                     unsigned long long *plan);
 
 int chz_version(void);
+
+/* Per-pass timing, the counterpart of the reference's TIME_TEST splits
+ * (e.g. cudasymmfft/dst1funcinplace.cu:145-218): with timing on, every pass
+ * of a transform / solve records an event at its start on the call's stream
+ * (and each call one at its end).  chz_get_timing waits for the events
+ * recorded since timing was switched on or last read, writes the pass
+ * durations (ms) and their comma-separated names, clears them, and returns
+ * the pass count. */
+void chz_set_timing(int on);
+int chz_get_timing(double *ms, int max_passes, char *names, int names_len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,157 @@
+"""GPU parity of the FP64 Helmholtz solve (HIP, through the C ABI).
+
+Bar: 1e-12 relative L-infinity against the oracle's restatement of
+cuda_helmholtz_solver, against sparse direct solves (golden fixtures) and
+against the hw3crt-contract CPU solver; O(h^2) against the analytic
+manufactured solution; at the full benchmark sizes (512^3, 512x384x256) the
+normalised FP64 7-point residual |f - A u| / (|A| |u| + |f|) < 1e-13.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not torch.cuda.is_available(), reason="no GPU")]
+
+import cuhelmholtz_amd as C  # noqa: E402
+from cuhelmholtz_amd import workloads as W  # noqa: E402
+from cuhelmholtz_amd import checks as H  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-12
+
+
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def faces(rng, n, bc):
+    NX, NY, NZ = n
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    return [rng.standard_normal(sizes[i]) if bc[i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None
+            for i in range(6)]
+
+
+def gpu_solve(f, n, bc, lam, bcs, dom=(0.0, 1.0) * 3):
+    ft = torch.from_numpy(np.ascontiguousarray(f)).to(dev())
+    C.helmholtz_solve(ft, n, bc, lam, bcs, dom)
+    torch.cuda.synchronize()
+    return ft.cpu().numpy()
+
+
+def test_golden_sparse_direct():
+    g = np.load(os.path.join(GOLD, "golden_solver.npz"))
+    n = tuple(int(v) for v in g["n"])
+    for t in range(int(g["count"])):
+        bc = tuple(int(b) for b in g[f"bc_{t}"])
+        lam = float(g[f"lam_{t}"])
+        bcs = [g[f"{nm}_{t}"] if f"{nm}_{t}" in g else None for nm in ["bcl", "bcr", "bcb", "bct", "bcf", "bce"]]
+        got = gpu_solve(g[f"f_{t}"], n, bc, lam, bcs)
+        want = g[f"u_{t}"] if f"u_{t}" in g else O.helmholtz(g[f"f_{t}"], n, bc, lam, bcs)
+        assert rel(got, want) < TOL, (bc, lam)
+
+
+@pytest.mark.parametrize("n,dom", [((6, 8, 10), (0.0, 1.0) * 3), ((12, 4, 6), (0.0, 2.0, -1.0, 0.5, 0.25, 1.0)),
+                                   ((16, 24, 8), (0.0, 1.0) * 3)])
+def test_all_bc_vs_oracle(n, dom):
+    rng = np.random.default_rng(1)
+    for bc in itertools.product(range(5), repeat=3):
+        f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+        bcs = faces(rng, n, bc)
+        got = gpu_solve(f, n, bc, -2.3, bcs, dom)
+        want = O.helmholtz(f, n, bc, -2.3, bcs, dom)
+        assert rel(got, want) < TOL, bc
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (3, 3, 3), (0, 3, 0), (3, 0, 3)])
+def test_singular_zero_mode(bc):
+    rng = np.random.default_rng(2)
+    n = (8, 12, 6)
+    f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+    bcs = faces(rng, n, bc)
+    got = gpu_solve(f, n, bc, 0.0, bcs)
+    assert rel(got, O.helmholtz(f, n, bc, 0.0, bcs)) < TOL
+    v, ierr, _ = O.solver_3d(f, n, bc, 0.0, bcs)
+    assert rel(got, v) < TOL
+
+
+def test_manufactured_vs_hw3crt_and_second_order():
+    """BASELINE configuration 1 (64^3, codes (1,2,0), lambda -1) against the
+    hw3crt-contract CPU solver at 1e-12, and O(h^2) against the analytic u."""
+    errs = []
+    for N in (32, 64, 128):
+        f, u, bcs = W.manufactured((N, N, N))
+        got = gpu_solve(f, (N, N, N), (1, 2, 0), -1.0, bcs)
+        if N == 64:
+            v, ierr, _ = O.solver_3d(f, (N, N, N), (1, 2, 0), -1.0, bcs, threads=8)
+            assert ierr == 0 and rel(got, v) < TOL
+        errs.append(np.max(np.abs(got - u)))
+    assert 3.6 < errs[0] / errs[1] < 4.4 and 3.6 < errs[1] / errs[2] < 4.4, errs
+
+
+@pytest.mark.parametrize("n,bc", [((128, 128, 128), (3, 4, 0)), ((256, 192, 128), (2, 1, 3))])
+def test_vs_hw3crt_medium(n, bc):
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+    bcs = faces(rng, n, bc)
+    got = gpu_solve(f, n, bc, -4.0, bcs)
+    v, ierr, _ = O.solver_3d(f, n, bc, -4.0, bcs, threads=16)
+    assert ierr == 0 and rel(got, v) < TOL
+
+
+@pytest.mark.parametrize("n", [(512, 512, 512), (512, 384, 256)])
+@pytest.mark.parametrize("lam", [-4.0, 0.5])
+def test_full_size_residual(n, lam):
+    """Mixed-precision workload shape, plain FP64 path: residual invariant."""
+    f, bcs = W.smooth_modes(n)
+    ft = torch.from_numpy(f).to(dev())
+    del f
+    ff = H.fold(ft, n, (3, 4, 0), bcs)
+    C.helmholtz_solve(ft, n, (3, 4, 0), lam, bcs)
+    r, scale = H.residual(ft, ff, n, (3, 4, 0), lam)
+    assert r / scale < 1e-13, (r, scale)
+    del ft, ff
+    torch.cuda.empty_cache()
+
+
+def test_errors_reported():
+    f = torch.zeros((9, 9, 9), dtype=torch.float64, device=dev())
+    with pytest.raises(C.CHZError, match="boundary condition"):
+        C.helmholtz_solve(f, (8, 8, 8), (5, 1, 1), -1.0)
+    import ctypes
+    L = C.lib()
+    rc = L.chz_helmholtz_solve(0.0, 1.0, 8, 3, None, None, 0.0, 1.0, 8, 1, None, None, 0.0, 1.0, 8, 1, None, None,
+                               -1.0, ctypes.c_void_p(f.data_ptr()), None)
+    assert rc == -1 and b"Neumann" in L.chz_last_error()
+
+
+def test_cpp_dropin_cuda_helmholtz_solver():
+    import ctypes
+    L = C.lib()
+    fn = getattr(L, "_Z21cuda_helmholtz_solverddiiPdS_ddiiS_S_ddiiS_S_dS_")
+    d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    fn.argtypes = [d, d, i, i, vp, vp] * 3 + [d, vp]
+    fn.restype = i
+    rng = np.random.default_rng(4)
+    n, bc = (16, 12, 8), (4, 0, 2)
+    f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+    bcs = faces(rng, n, bc)
+    want = O.helmholtz(f, n, bc, -1.0, bcs)
+    ft = torch.from_numpy(f).to(dev())
+    bt = [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs]
+    p = [None if b is None else vp(b.data_ptr()) for b in bt]
+    torch.cuda.synchronize()
+    rc = fn(0.0, 1.0, n[0], bc[0], p[0], p[1], 0.0, 1.0, n[1], bc[1], p[2], p[3], 0.0, 1.0, n[2], bc[2], p[4], p[5],
+            -1.0, vp(ft.data_ptr()))
+    assert rc == 0
+    assert rel(ft.cpu().numpy(), want) < TOL

@@ -16,7 +16,7 @@ import pytest
 import oracle as O
 
 torch = pytest.importorskip("torch")
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not torch.cuda.is_available(), reason="no GPU")]
 
 import cuhelmholtz_amd as C  # noqa: E402
 
@@ -173,7 +173,7 @@ def test_errors_are_reported():
     y = torch.zeros((8, 8, 8), dtype=torch.float64, device=dev())
     with pytest.raises(C.CHZError, match="kind"):
         C.do_transform(y, None, (9, O.DCT_2, O.DCT_2))
-    z = torch.zeros((8, 8, 12), dtype=torch.float64, device=dev())  # NX = 10: N/2 = 5 unsupported
+    z = torch.zeros((8, 8, 16), dtype=torch.float64, device=dev())  # NX = 14: N/2 = 7 unsupported
     with pytest.raises(C.CHZError, match="unsupported"):
         C.do_transform(z, None, (O.DCT_2, O.DCT_2, O.DCT_2))
 
