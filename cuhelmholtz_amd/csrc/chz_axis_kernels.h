@@ -12,17 +12,18 @@ __global__ void __launch_bounds__(256) k_axis_pass(PassArgs p)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
-    V *buf = reinterpret_cast<V *>(smem);
     const int L = p.L;
-    const Tile<T> tl{buf, L + 1, (int)threadIdx.x % L};
+    const Smem<T> sm(smem, p.M, L);
+    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
-    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
+    const V *tw = reinterpret_cast<const V *>(p.tw);
     const int a_tile = p.a0 + (int)blockIdx.x * L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const long long b = p.b0 + (long long)blockIdx.y;
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
     __syncthreads();
-    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, reinterpret_cast<const V *>(p.tw), red);
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     store_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
 }
 
@@ -86,19 +87,20 @@ __global__ void __launch_bounds__(256) k_solve_xfwd(PassArgs p, SolveArgs sa)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
-    V *buf = reinterpret_cast<V *>(smem);
     const int L = p.L;
-    const Tile<T> tl{buf, L + 1, (int)threadIdx.x % L};
+    const Smem<T> sm(smem, p.M, L);
+    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
-    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
+    const V *tw = reinterpret_cast<const V *>(p.tw);
     const int a_tile = p.a0 + (int)blockIdx.x * L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const int k = p.b0 + (int)blockIdx.y;
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
     __syncthreads();
     fold_boundaries<T>(sa, tl, nlines, a_tile, k);
     __syncthreads();
-    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, reinterpret_cast<const V *>(p.tw), red);
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
@@ -111,25 +113,26 @@ __global__ void __launch_bounds__(256) k_solve_zmid(PassArgs p, SolveArgs sa)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
-    V *buf = reinterpret_cast<V *>(smem);
     const int L = p.L;
     const int c = (int)threadIdx.x % L;
-    const Tile<T> tl{buf, L + 1, c};
+    const Smem<T> sm(smem, p.M, L);
+    const Tile<T> tl{sm.buf, L + 1, c};
     const int u = (int)threadIdx.x / L;
-    T *red = reinterpret_cast<T *>(buf + lds_points(p.M, L));
     const int a_tile = p.a0 + (int)blockIdx.x * L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const int y = p.b0 + (int)blockIdx.y;
     const V *tw = reinterpret_cast<const V *>(p.tw);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, false>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
     __syncthreads();
-    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, red);
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     {
         const int x = a_tile + c;
         const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
         const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
         const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
         T *rbuf = reinterpret_cast<T *>(tl.buf);
+#pragma unroll 4
         for (int s = p.out_lo + u; s <= p.out_hi; s += p.TPL) {
             T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
             const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
@@ -138,7 +141,7 @@ __global__ void __launch_bounds__(256) k_solve_zmid(PassArgs p, SolveArgs sa)
         }
     }
     __syncthreads();
-    line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, red);
+    line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     PassArgs q = p;
     q.out_lo = sa.out2_lo;
     q.out_hi = sa.out2_hi;

@@ -138,7 +138,8 @@ bool Context::config(int N, int prec, bool rows, AxisCfg *out)
     if (!c.EPT) return false;
     int L = 256 / c.TPL;
     if (L < 1) L = 1;
-    const size_t cap = lds_cap_ < 65536 ? lds_cap_ : 65536;
+    // two workgroups per CU: half of the per-CU LDS (160 KB on gfx950), at most 80 KB
+    const size_t cap = lds_cap_ / 2 < 81920 ? (lds_cap_ / 2 > 32768 ? lds_cap_ / 2 : 32768) : 81920;
     while (L > 1 && axis_pass_lds(prec, M, L, c.TPL) > cap) L--;
     if (axis_pass_lds(prec, M, L, c.TPL) > lds_cap_) return false;
     c.L = L;

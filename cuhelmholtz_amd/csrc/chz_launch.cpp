@@ -6,7 +6,8 @@ namespace chz {
 size_t axis_pass_lds(int prec, int M, int L, int TPL)
 {
     const size_t esz = prec == 64 ? 8 : 4;
-    return (size_t)lds_points(M, L) * 2 * esz + (size_t)TPL * L * esz;
+    // tile points + FFT twiddles (complex) + reduction scratch (real)
+    return ((size_t)lds_points(M, L) + (size_t)M) * 2 * esz + (size_t)TPL * L * esz;
 }
 
 hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s)

@@ -72,6 +72,12 @@ __device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, i
             for (int j = u; j < M; j += TPL) s += tl.re(2 * j + 1);
             red[u * (tl.LS - 1) + tl.c] = s;
         }
+        V wk[PP];
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);
+        }
 #pragma unroll
         for (int i = 0; i < PP; i++) {
             const int k = u + i * TPL;
@@ -79,7 +85,7 @@ __device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, i
                 V Hk = hval<KIND, T>(tl, k, M, N), Hm = hval<KIND, T>(tl, M - k, M, N);
                 V E = mk<V>(Hk.x + Hm.x, Hk.y - Hm.y);
                 V D = mk<V>(Hk.x - Hm.x, Hk.y + Hm.y);
-                V O = cmul(D, tw[4 * k]);
+                V O = cmul(D, wk[i]);
                 ca[i] = mk<V>(E.x - O.y, -(E.y + O.x));
                 cb[i] = mk<V>(E.x + O.y, E.y - O.x);
             }
@@ -97,12 +103,17 @@ __device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, i
         // DCT_3 / DST_3 pre-rotation of the pairs (t, N-t), then the real line
         // itself is the FFT input (point m = x'_{2m} + i x'_{2m+1}).
         T va[PT], vb[PT];
-        const V *half = tw;
+        V csv[PT];
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int t = u + i * TPL;
+            csv[i] = t <= M ? tw[t] : mk<V>(0, 0);   // (cos, sin)(t pi / 2N)
+        }
 #pragma unroll
         for (int i = 0; i < PT; i++) {
             const int t = u + i * TPL;
             if (t <= M) {
-                const V cs = half[t];      // (cos, sin)(t pi / 2N)
+                const V cs = csv[i];
                 if constexpr (KIND == K_DCT_3) {   // dct3funcinplace.cu:53-64 (its t+1 is t here)
                     if (t >= 1) {
                         const T a = tl.re(t), b = tl.re(N - t);
@@ -157,8 +168,14 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
             const int L = tl.LS - 1;
             for (int v = 0; v < TPL; v++) x1 += red[v * L + tl.c];
         }
-        const V *half = tw;
-        const V *rsin = tw + TwOffsets::rsin(N);
+        // per-n table value: (cos, sin)(n pi/2N) for DCT_2 / DST_2, (1/sin(n pi/N), 0) for DST_1 / DCT_1
+        V tab[PT];
+        const V *tsrc = (KIND == K_DCT_2 || KIND == K_DST_2) ? tw : tw + TwOffsets::rsin(N);
+#pragma unroll
+        for (int i = 0; i < PT; i++) {
+            const int n = u + i * TPL;
+            tab[i] = (KIND != K_C2R && n <= M) ? tsrc[n] : mk<V>(0, 0);
+        }
 #pragma unroll
         for (int i = 0; i < PT; i++) {
             const int n = u + i * TPL;
@@ -168,26 +185,26 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
                 if constexpr (KIND == K_DST_1) {          // dst1funcinplace.cu:97-107
                     if (n == 0) { oa[i] = 0; ob[i] = 0; }
                     else {
-                        const T Ta = (a + b) * rsin[n].x * (T)0.25, Tb = (a - b) * h;
+                        const T Ta = (a + b) * tab[i].x * (T)0.25, Tb = (a - b) * h;
                         oa[i] = (Ta + Tb) * h; ob[i] = (Ta - Tb) * h;
                     }
                 } else if constexpr (KIND == K_DCT_1) {   // dct1funcinplace.cu:116-127
                     if (n == 0) { oa[i] = a * h + x1; ob[i] = a * h - x1; }
                     else {
-                        const T Ta = (a + b) * h, Tb = (a - b) * (T)0.25 * rsin[n].x;
+                        const T Ta = (a + b) * h, Tb = (a - b) * (T)0.25 * tab[i].x;
                         oa[i] = (Ta - Tb) * h; ob[i] = (Ta + Tb) * h;
                     }
                 } else if constexpr (KIND == K_DCT_2) {   // dct2funcinplace.cu:93-105 (out[0] = y_0)
                     if (n == 0) { oa[i] = a; ob[i] = 0; }
                     else {
-                        const V cs = half[n];
+                        const V cs = tab[i];
                         const T Ta = a + b, Tb = a - b;
                         oa[i] = (Ta * cs.x + Tb * cs.y) * h; ob[i] = (Ta * cs.y - Tb * cs.x) * h;
                     }
                 } else if constexpr (KIND == K_DST_2) {   // dst2funcinplace.cu:89-103 (out[0] = 0, out[N] = y_0)
                     if (n == 0) { oa[i] = 0; ob[i] = a; }
                     else {
-                        const V cs = half[n];
+                        const V cs = tab[i];
                         const T Ta = a + b, Tb = a - b;
                         oa[i] = (Ta * cs.y + Tb * cs.x) * h; ob[i] = (Ta * cs.x - Tb * cs.y) * h;
                     }
@@ -214,7 +231,12 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
         // D2Z split of the M-point FFT Z into the N-point spectrum X_k (k = 0..M):
         // E_k = (Z_k + conj Z_{M-k})/2, O_k = -i (Z_k - conj Z_{M-k})/2,
         // X_k = E_k + conj(w^k) O_k, X_{M-k} = conj(E_k - conj(w^k) O_k).
-        V xa[PP], xb[PP];
+        V xa[PP], xb[PP], wk[PP];
+#pragma unroll
+        for (int i = 0; i < PP; i++) {
+            const int k = u + i * TPL;
+            wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);
+        }
 #pragma unroll
         for (int i = 0; i < PP; i++) {
             const int k = u + i * TPL;
@@ -222,7 +244,7 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
                 const V Zk = tl.pt(k), Zm = tl.pt(k == 0 ? 0 : M - k);
                 const V E = mk<V>(h * (Zk.x + Zm.x), h * (Zk.y - Zm.y));
                 const V O = mk<V>(h * (Zk.y + Zm.y), -h * (Zk.x - Zm.x));
-                const V w = tw[4 * k];
+                const V w = wk[i];
                 const V P = mk<V>(w.x * O.x + w.y * O.y, w.x * O.y - w.y * O.x);   // conj(w) O
                 xa[i] = mk<V>(E.x + P.x, E.y + P.y);
                 xb[i] = mk<V>(E.x - P.x, -(E.y - P.y));
@@ -265,54 +287,77 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
 
 
 
+// Loads in flight per thread while staging a tile (the loads of a chunk are
+// all issued before the first LDS write, so a thread waits once per chunk
+// rather than once per element).
+template <typename T> struct StageChunk { static constexpr int value = sizeof(T) == 8 ? 24 : 32; };
+
 // Load the tile into LDS.  ROWS: lines are consecutive rows of a row-major
 // volume (x-pass), the tile is one contiguous run of memory.  !ROWS: lines
 // are L consecutive x-columns (y/z passes), slot stride s_sp.
 template <typename T, bool ROWS, typename SRC>
 __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, const SRC *src)
 {
+    constexpr int CH = StageChunk<T>::value;
     const int t = threadIdx.x, nt = blockDim.x;
     const int NS = p.N + 2;
     const int L = p.L;
+    T *rbuf = reinterpret_cast<T *>(tl.buf);
+    const int LS = tl.LS;
     if (ROWS) {
         const int RL = p.s_rowlen;
         const int total = nlines * RL;
         const SRC *base = src + lbase;
-        T *rbuf = reinterpret_cast<T *>(tl.buf);
-        const int LS = tl.LS;
         // incremental (row, slot) of idx = t + j*nt
         int r = t / RL, s = t - (t / RL) * RL;
         const int dq = nt / RL, dr = nt - (nt / RL) * RL;
-        for (int idx = t; idx < total; idx += nt) {
-            if (s < NS) {
-                T v = 0;
-                if (s >= p.in_lo && s <= p.in_hi) v = (T)base[idx];
-                rbuf[(size_t)((s >> 1) * LS + r) * 2 + (s & 1)] = v;
+        for (int i0 = t; i0 < total; i0 += CH * nt) {
+            T v[CH];
+            int rs[CH], ss[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int idx = i0 + i * nt;
+                rs[i] = r;
+                ss[i] = s;
+                v[i] = 0;
+                if (idx < total && s < NS && s >= p.in_lo && s <= p.in_hi) v[i] = (T)base[idx];
+                s += dr; r += dq;
+                if (s >= RL) { s -= RL; r++; }
             }
-            s += dr; r += dq;
-            if (s >= RL) { s -= RL; r++; }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                if (i0 + i * nt < total && ss[i] < NS) rbuf[(size_t)((ss[i] >> 1) * LS + rs[i]) * 2 + (ss[i] & 1)] = v[i];
+            }
         }
         // slots beyond the source row, and lines beyond the tile end, are 0
         if (RL < NS) {
             const int extra = NS - RL;
             for (int idx = t; idx < nlines * extra; idx += nt) {
-                const int rr = idx / extra, ss = RL + idx - (idx / extra) * extra;
-                rbuf[(size_t)((ss >> 1) * LS + rr) * 2 + (ss & 1)] = 0;
+                const int rr = idx / extra, sx = RL + idx - (idx / extra) * extra;
+                rbuf[(size_t)((sx >> 1) * LS + rr) * 2 + (sx & 1)] = 0;
             }
         }
         for (int idx = t; idx < (L - nlines) * NS; idx += nt) {
-            const int rr = nlines + idx / NS, ss = idx - (idx / NS) * NS;
-            rbuf[(size_t)((ss >> 1) * LS + rr) * 2 + (ss & 1)] = 0;
+            const int rr = nlines + idx / NS, sx = idx - (idx / NS) * NS;
+            rbuf[(size_t)((sx >> 1) * LS + rr) * 2 + (sx & 1)] = 0;
         }
     } else {
         const int c = t % L, u0 = t / L, step = nt / L;
         const bool ok = c < nlines;
         const SRC *base = src + lbase + (long long)c * p.s_sa;
-        T *rbuf = reinterpret_cast<T *>(tl.buf);
-        for (int s = u0; s < NS; s += step) {
-            T v = 0;
-            if (ok && s >= p.in_lo && s <= p.in_hi) v = (T)base[(long long)s * p.s_sp];
-            rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)] = v;
+        for (int s0 = u0; s0 < NS; s0 += CH * step) {
+            T v[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int s = s0 + i * step;
+                v[i] = 0;
+                if (ok && s < NS && s >= p.in_lo && s <= p.in_hi) v[i] = (T)base[(long long)s * p.s_sp];
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int s = s0 + i * step;
+                if (s < NS) rbuf[(size_t)((s >> 1) * LS + c) * 2 + (s & 1)] = v[i];
+            }
         }
     }
 }
@@ -320,6 +365,7 @@ __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, 
 template <typename T, bool ROWS, typename DST>
 __device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, DST *dst)
 {
+    constexpr int CH = StageChunk<T>::value;
     const int t = threadIdx.x, nt = blockDim.x;
     const T sc = (T)p.scale;
     const T *rbuf = reinterpret_cast<const T *>(tl.buf);
@@ -329,52 +375,91 @@ __device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl,
         DST *base = dst + lbase;
         int r = t / RL, s = t - (t / RL) * RL;
         const int dq = nt / RL, dr = nt - (nt / RL) * RL;
-        for (int idx = t; idx < total; idx += nt) {
-            if (s >= p.st_lo && s <= p.st_hi) {
-                T v = 0;
-                if (s >= p.out_lo && s <= p.out_hi) v = rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] * sc;
-                base[idx] = (DST)v;
+        for (int i0 = t; i0 < total; i0 += CH * nt) {
+            T v[CH];
+            bool w[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                w[i] = i0 + i * nt < total && s >= p.st_lo && s <= p.st_hi;
+                v[i] = 0;
+                if (w[i] && s >= p.out_lo && s <= p.out_hi) v[i] = rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] * sc;
+                s += dr; r += dq;
+                if (s >= RL) { s -= RL; r++; }
             }
-            s += dr; r += dq;
-            if (s >= RL) { s -= RL; r++; }
+#pragma unroll
+            for (int i = 0; i < CH; i++)
+                if (w[i]) base[i0 + i * nt] = (DST)v[i];
         }
     } else {
         const int L = p.L;
         const int c = t % L, u0 = t / L, step = nt / L;
         if (c >= nlines) return;
         DST *base = dst + lbase + (long long)c * p.d_sa;
-        for (int s = p.st_lo + u0; s <= p.st_hi; s += step) {
-            T v = 0;
-            if (s >= p.out_lo && s <= p.out_hi) v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)] * sc;
-            base[(long long)s * p.d_sp] = (DST)v;
+        for (int s0 = p.st_lo + u0; s0 <= p.st_hi; s0 += CH * step) {
+            T v[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int s = s0 + i * step;
+                v[i] = 0;
+                if (s <= p.st_hi && s >= p.out_lo && s <= p.out_hi) v[i] = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)] * sc;
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int s = s0 + i * step;
+                if (s <= p.st_hi) base[(long long)s * p.d_sp] = (DST)v[i];
+            }
         }
     }
 }
 
+// Copy the M-point FFT twiddles of the pass from global memory into LDS.
+template <typename V>
+__device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M)
+{
+    for (int i = threadIdx.x; i < M; i += blockDim.x) twl[i] = gtw[i];
+}
+
+// Shared-memory carve-up of a pass: tile points, FFT twiddles, reduction scratch.
+template <typename T> struct Smem {
+    typedef typename vec2_of<T>::type V;
+    V *buf;
+    V *twl;
+    T *red;
+    __device__ __forceinline__ Smem(unsigned char *smem, int M, int L)
+    {
+        buf = reinterpret_cast<V *>(smem);
+        twl = buf + lds_points(M, L);
+        red = reinterpret_cast<T *>(twl + M);
+    }
+};
+
 // One line transform of the whole tile (fold, FFT, unfold), data in LDS.
 template <int KIND, int EPT, typename T>
 __device__ __forceinline__ void line_transform_k(const Tile<T> &tl, int u, int TPL, int M, int N,
-                                                 unsigned long long plan, const typename vec2_of<T>::type *tw, T *red)
+                                                 unsigned long long plan, const typename vec2_of<T>::type *tw,
+                                                 const typename vec2_of<T>::type *twl, T *red)
 {
     fold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
-    fft_forward<EPT, T>(tl, u, TPL, M, plan, tw + TwOffsets::fftw(N));
+    fft_forward<EPT, T>(tl, u, TPL, M, plan, twl);
     unfold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
 }
 
 // Runtime kind -> one specialised path (the branch is uniform over the workgroup).
+// tw: the pass's twiddle table in global memory; twl: its FFT part staged in LDS.
 template <int EPT, typename T>
 __device__ __forceinline__ void line_transform(int kind, const Tile<T> &tl, int u, int TPL, int M, int N,
-                                               unsigned long long plan, const typename vec2_of<T>::type *tw, T *red)
+                                               unsigned long long plan, const typename vec2_of<T>::type *tw,
+                                               const typename vec2_of<T>::type *twl, T *red)
 {
     switch (kind) {
-        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_DST_1: line_transform_k<K_DST_1, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_DST_2: line_transform_k<K_DST_2, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_DST_3: line_transform_k<K_DST_3, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        case K_R2C: line_transform_k<K_R2C, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
-        default: line_transform_k<K_C2R, EPT, T>(tl, u, TPL, M, N, plan, tw, red); break;
+        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_1: line_transform_k<K_DST_1, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_2: line_transform_k<K_DST_2, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_3: line_transform_k<K_DST_3, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_R2C: line_transform_k<K_R2C, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        default: line_transform_k<K_C2R, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
     }
 }
 
