@@ -45,6 +45,8 @@ def lib():
         L.chz_do_transform_f32.argtypes = [vp, vp, i, i, i, i, i, i, vp]
         L.chz_last_error.restype = ctypes.c_char_p
         L.chz_set_timing.argtypes = [i]
+        L.chz_set_axis_config.argtypes = [i] * 6
+        L.chz_load_tuning.argtypes = [ctypes.c_char_p]
         L.chz_get_timing.argtypes = [ctypes.POINTER(dp), i, ctypes.c_char_p, i]
         L.chz_axis_config.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                       ctypes.POINTER(ctypes.c_ulonglong)]
@@ -168,10 +170,17 @@ def get_timing(max_passes=4096):
     return list(zip(names.value.decode().split(",")[:n], [ms[i] for i in range(n)]))
 
 
-def axis_config(N, prec=64, rows=True):
+ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS = range(5)
+ROLE_NAMES = ["rows", "cols", "xfwd", "zmid", "zcols"]
+
+
+def axis_config(N, prec=64, rows=True, role=None):
+    """Launch configuration of a pass kernel (role defaults to rows/cols)."""
+    if role is None:
+        role = ROLE_ROWS if rows else ROLE_COLS
     e, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     p = ctypes.c_ulonglong()
-    _check(lib().chz_axis_config(N, prec, 1 if rows else 0, ctypes.byref(e), ctypes.byref(t), ctypes.byref(l),
+    _check(lib().chz_axis_config(N, prec, role, ctypes.byref(e), ctypes.byref(t), ctypes.byref(l),
                                  ctypes.byref(p)))
     radices = []
     v = p.value
@@ -179,6 +188,15 @@ def axis_config(N, prec=64, rows=True):
         radices.append(v & 255)
         v >>= 8
     return {"ept": e.value, "tpl": t.value, "lines": l.value, "radices": radices}
+
+
+def set_axis_config(N, prec, role, ept, tpl, lines):
+    """Pin a launch configuration; raises CHZError if it cannot run."""
+    _check(lib().chz_set_axis_config(N, prec, role, ept, tpl, lines))
+
+
+def reset_axis_configs():
+    lib().chz_reset_axis_configs()
 
 
 def in_window(kind, N):

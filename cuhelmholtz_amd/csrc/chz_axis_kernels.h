@@ -8,7 +8,7 @@ namespace chz {
 
 // One fused axis pass over a tile of L lines: load -> fold -> FFT -> unfold -> store.
 template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
-__global__ void __launch_bounds__(256) k_axis_pass(PassArgs p)
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -17,9 +17,10 @@ __global__ void __launch_bounds__(256) k_axis_pass(PassArgs p)
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    int a_tile, bi;
+    tile_of(p, a_tile, bi);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    const long long b = p.b0 + (long long)blockIdx.y;
+    const long long b = bi;
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
     __syncthreads();
@@ -83,7 +84,7 @@ __device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<
 // Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
 // folds, forward x transform, store into the padded spectral workspace.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_solve_xfwd(PassArgs p, SolveArgs sa)
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, SolveArgs sa)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -92,9 +93,9 @@ __global__ void __launch_bounds__(256) k_solve_xfwd(PassArgs p, SolveArgs sa)
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    int a_tile, k;
+    tile_of(p, a_tile, k);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    const int k = p.b0 + (int)blockIdx.y;
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
     __syncthreads();
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(256) k_solve_xfwd(PassArgs p, SolveArgs sa)
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
 // inverse z transform, all in LDS.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_solve_zmid(PassArgs p, SolveArgs sa)
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, SolveArgs sa)
 {
     typedef typename vec2_of<T>::type V;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -118,9 +119,9 @@ __global__ void __launch_bounds__(256) k_solve_zmid(PassArgs p, SolveArgs sa)
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, c};
     const int u = (int)threadIdx.x / L;
-    const int a_tile = p.a0 + (int)blockIdx.x * L;
+    int a_tile, y;
+    tile_of(p, a_tile, y);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    const int y = p.b0 + (int)blockIdx.y;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, false>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));

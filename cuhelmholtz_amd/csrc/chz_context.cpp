@@ -1,9 +1,14 @@
 // chz_context.cpp -- twiddle tables, axis configurations, workspaces, errors.
 #include "chz_context.h"
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
 #include <tuple>
 
 namespace chz {
@@ -42,6 +47,34 @@ bool radix_plan(int M, int ept, int tpl, unsigned long long *plan)
     return true;
 }
 
+int Context::load_tuning(const char *path)
+{
+    std::ifstream in(path);
+    if (!in) return -1;
+    std::string line;
+    int n = 0;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        int role, N, prec, ept, tpl, lines;
+        if (!(ss >> role >> N >> prec >> ept >> tpl >> lines)) continue;
+        if (set_config(N, prec, role, ept, tpl, lines, nullptr)) n++;
+    }
+    return n;
+}
+
+// directory of this shared library (for the tuning tables shipped beside it)
+static std::string library_dir()
+{
+    Dl_info info;
+    if (dladdr((void *)&library_dir, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t k = p.rfind('/');
+        if (k != std::string::npos) return p.substr(0, k);
+    }
+    return ".";
+}
+
 Context::Context(int dev) : dev_(dev)
 {
     int v = 0;
@@ -51,6 +84,21 @@ Context::Context(int dev) : dev_(dev)
         size_t c = (size_t)atol(e);
         if (c > 0 && c < lds_cap_) lds_cap_ = c;
     }
+    // tuned launch configurations: CHZ_TUNE_FILE, else tuning/<arch>.txt beside the library
+    std::string path;
+    if (const char *e = getenv("CHZ_TUNE_FILE")) {
+        path = e;
+    } else {
+        hipDeviceProp_t prop;
+        std::string arch = "gfx950";
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            arch = prop.gcnArchName;
+            size_t k = arch.find(':');
+            if (k != std::string::npos) arch = arch.substr(0, k);
+        }
+        path = library_dir() + "/tuning/" + arch + ".txt";
+    }
+    if (!getenv("CHZ_NO_TUNING")) load_tuning(path.c_str());
 }
 
 Context &Context::get()
@@ -111,22 +159,41 @@ const void *Context::twiddles(int N, int prec)
     return d;
 }
 
-bool Context::config(int N, int prec, bool rows, AxisCfg *out)
+bool Context::set_config(int N, int prec, int role, int ept, int tpl, int lines, unsigned long long *plan_out)
 {
-    auto key = std::make_tuple(N, prec, rows);
+    static const int epts[] = {16, 12, 10, 8, 6, 5, 4};
+    bool ok = false;
+    for (int e : epts) ok |= e == ept;
+    const int M = N / 2;
+    if (!ok || N < 2 || (N & 1) || tpl < 1 || lines < 1 || tpl * lines > CHZ_MAX_THREADS || M > ept * tpl) return false;
+    if (role < 0 || role >= ROLE_COUNT || (prec != 64 && prec != 32)) return false;
+    unsigned long long p;
+    if (!radix_plan(M, ept, tpl, &p)) return false;
+    if (axis_pass_lds(prec, M, lines, tpl) > lds_cap_) return false;
+    AxisCfg c;
+    c.N = N; c.M = M; c.EPT = ept; c.TPL = tpl; c.L = lines; c.plan = p;
+    cfg_[std::make_tuple(N, prec, role)] = c;
+    if (plan_out) *plan_out = p;
+    return true;
+}
+
+bool Context::config(int N, int prec, int role, AxisCfg *out)
+{
+    auto key = std::make_tuple(N, prec, role);
     auto it = cfg_.find(key);
     if (it != cfg_.end()) {
         *out = it->second;
         return true;
     }
     const int M = N / 2;
-    static const int epts[] = {16, 12, 10, 8, 6, 5, 4};
+    // 8 points per thread first (measured best for FP64 on gfx950), then larger / smaller
+    static const int epts[] = {8, 16, 12, 10, 6, 5, 4};
     AxisCfg c;
     c.N = N;
     c.M = M;
     for (int e : epts) {
         if (c.EPT) break;
-        for (int tpl = (M + e - 1) / e; tpl <= 256 && tpl <= (M > 1 ? M : 1); tpl++) {
+        for (int tpl = (M + e - 1) / e; tpl <= CHZ_MAX_THREADS && tpl <= (M > 1 ? M : 1); tpl++) {
             unsigned long long p;
             if (!radix_plan(M, e, tpl, &p)) continue;
             c.EPT = e;

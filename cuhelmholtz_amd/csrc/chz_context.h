@@ -18,6 +18,9 @@ struct AxisCfg {
     unsigned long long plan = 0;   // radices, one byte per stage
 };
 
+// Kernel roles that keep separate launch configurations.
+enum Role { ROLE_ROWS = 0, ROLE_COLS = 1, ROLE_XFWD = 2, ROLE_ZMID = 3, ROLE_ZCOLS = 4, ROLE_COUNT = 5 };
+
 // Status codes of the C ABI (also in include/cuhelmholtz.h).
 enum Status { OK = 0, ERR_INVALID = -1, ERR_UNSUPPORTED = -2, ERR_DEVICE = -3, ERR_ALLOC = -4 };
 
@@ -30,8 +33,14 @@ public:
     static Context &get();           // context of the current device
     // device twiddle table of N at precision prec (64 or 32); nullptr on failure
     const void *twiddles(int N, int prec);
-    // axis-pass configuration; rows = x-pass row tiles
-    bool config(int N, int prec, bool rows, AxisCfg *out);
+    // launch configuration of a pass kernel (role: see Role)
+    bool config(int N, int prec, int role, AxisCfg *out);
+    bool config(int N, int prec, bool rows, AxisCfg *out) { return config(N, prec, rows ? ROLE_ROWS : ROLE_COLS, out); }
+    // validate and pin a configuration (autotuner); false if it cannot run
+    bool set_config(int N, int prec, int role, int ept, int tpl, int lines, unsigned long long *plan_out);
+    void reset_configs() { cfg_.clear(); }
+    // pin the configurations listed in a tuning table ("role N prec ept tpl lines" per line)
+    int load_tuning(const char *path);
     size_t lds_cap() const { return lds_cap_; }
     int device() const { return dev_; }
     void release();
@@ -56,7 +65,7 @@ private:
     size_t lds_cap_ = 65536;
     std::mutex mu_;
     std::map<std::pair<int, int>, void *> tw_;
-    std::map<std::tuple<int, int, bool>, AxisCfg> cfg_;
+    std::map<std::tuple<int, int, int>, AxisCfg> cfg_;
     std::map<int, std::pair<void *, size_t>> ws_;
     std::map<std::string, void *> blobs_;
     bool timing_ = false;

@@ -7,6 +7,9 @@
 
 namespace chz {
 
+// largest workgroup of a pass kernel (lines per tile x threads per line)
+#define CHZ_MAX_THREADS 512
+
 enum Kind { K_DCT_1 = 0, K_DCT_2, K_DCT_3, K_DST_1, K_DST_2, K_DST_3, K_R2C, K_C2R };
 
 __host__ __device__ constexpr bool kind_is_z2d(int k) { return k == K_DST_1 || k == K_DCT_1 || k == K_DCT_2 || k == K_DST_2 || k == K_C2R; }
@@ -34,6 +37,7 @@ struct PassArgs {
     int st_lo, st_hi;             // slots stored
     int out_lo, out_hi;           // valid output slots (others stored as 0)
     int L, TPL;
+    int xcds;                     // XCD count for the tile swizzle (1 = dispatch order)
     unsigned long long plan;
     const void *tw;
     double scale;

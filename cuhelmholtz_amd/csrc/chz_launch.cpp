@@ -15,7 +15,7 @@ hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int
     const void *fn = prec == 64 ? (rows ? axis_kernel_f64_rows(ept) : axis_kernel_f64_cols(ept))
                                 : (rows ? axis_kernel_f32_rows(ept) : axis_kernel_f32_cols(ept));
     if (!fn) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((p.na + p.L - 1) / p.L), (unsigned)nb, 1);
+    dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
     PassArgs a = p;
     void *args[] = {&a};
     return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
@@ -24,7 +24,7 @@ hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int
 hipError_t launch_solve_pass(const void *fn, const PassArgs &p, const SolveArgs &sa, int prec, int nb, hipStream_t s)
 {
     if (!fn) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((p.na + p.L - 1) / p.L), (unsigned)nb, 1);
+    dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
     PassArgs a = p;
     SolveArgs b = sa;
     void *args[] = {&a, &b};

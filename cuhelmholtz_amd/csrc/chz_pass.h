@@ -412,6 +412,28 @@ __device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl,
     }
 }
 
+// Tile of this workgroup.  The grid is 1-D over (tiles along a) x (lines b);
+// workgroups are dispatched round-robin over the XCDs, each with its own L2,
+// so the linear id is swizzled to hand every XCD a contiguous range of
+// logical tiles: neighbouring tiles, which share cache lines at their edges,
+// then run on the same XCD at about the same time.
+__device__ __forceinline__ void tile_of(const PassArgs &p, int &a_tile, int &b)
+{
+    const int ntiles = (p.na + p.L - 1) / p.L;
+    const int total = (int)gridDim.x;
+    const int w = (int)blockIdx.x;
+    int logical = w;
+    const int X = p.xcds;
+    if (X > 1 && total >= X) {
+        const int per = (total + X - 1) / X;          // tiles of the first `tall` XCDs
+        const int tall = total - (per - 1) * X;       // XCDs that get `per` tiles
+        const int xcd = w % X, local = w / X;
+        logical = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+    }
+    a_tile = p.a0 + (logical % ntiles) * p.L;
+    b = p.b0 + logical / ntiles;
+}
+
 // Copy the M-point FFT twiddles of the pass from global memory into LDS.
 template <typename V>
 __device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M)

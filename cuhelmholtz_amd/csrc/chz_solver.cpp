@@ -24,6 +24,15 @@
 
 namespace chz {
 
+// Row length of the solve workspace: the N+2 slots of the padded layout rounded
+// up to whole 128-byte lines, so every row and plane starts on a cache line
+// and the column tiles of the y/z passes touch whole lines only.
+long long ws_row(int NX, size_t esz)
+{
+    const long long per_line = 128 / (long long)esz;
+    return ((NX + 2 + per_line - 1) / per_line) * per_line;
+}
+
 // bc code -> unknown window and transform kinds (cuda_helmholtz_solver.cu:78-172)
 static bool bc_kinds(int bc, int N, int *s, int *e, int *fwd, int *inv)
 {
@@ -122,13 +131,13 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx, cy, cz;
-    if (!ctx.config(NX, prec, true, &cx) || !ctx.config(NY, prec, false, &cy) || !ctx.config(NZ, prec, false, &cz))
+    if (!ctx.config(NX, prec, ROLE_XFWD, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) || !ctx.config(NZ, prec, ROLE_ZMID, &cz))
         return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
     const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
     const double *eig = solver_eig(ctx, g);
     if (!twx || !twy || !twz || !eig) return fail(ERR_ALLOC, "table allocation failed");
 
-    const long long fx = NX + 1, fy = NY + 1, P0 = NX + 2, P1 = NY + 2;
+    const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(T)), P1 = NY + 2;
     const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
     int sxl, sxh, syl, syh, szl, szh;   // spectral windows (forward outputs)
     out_window(g.fwd[0], NX, &sxl, &sxh);
@@ -218,9 +227,9 @@ int solve_store(Context &ctx, const SolveGeom &g, double *f, const T *ex, hipStr
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx;
-    if (!ctx.config(NX, prec, true, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
+    if (!ctx.config(NX, prec, ROLE_ROWS, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
     const void *twx = ctx.twiddles(NX, prec);
-    const long long fx = NX + 1, fy = NY + 1, P0 = NX + 2, P1 = NY + 2;
+    const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(T)), P1 = NY + 2;
     const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
     int sxl, sxh;
     out_window(g.fwd[0], NX, &sxl, &sxh);
@@ -263,7 +272,7 @@ int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const d
     if (rc) return rc;
     Context &ctx = Context::get();
     std::lock_guard<std::mutex> lock(ctx.mutex());
-    const size_t bytes = sizeof(double) * (size_t)(N[0] + 2) * (N[1] + 2) * (N[2] + 2);
+    const size_t bytes = sizeof(double) * (size_t)ws_row(N[0], sizeof(double)) * (N[1] + 2) * (N[2] + 2);
     double *ex = reinterpret_cast<double *>(ctx.workspace(bytes, 0));
     if (!ex) return fail(ERR_ALLOC, "workspace allocation failed");
     PassTimer timer(ctx, st);

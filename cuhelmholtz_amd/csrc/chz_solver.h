@@ -16,6 +16,7 @@ struct SolveGeom {
     const double *faces[6];   // bcl, bcr, bcb, bct, bcf, bce (null where unused)
 };
 
+long long ws_row(int NX, size_t esz);
 int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3], double lambda,
               const double *const faces[6]);
 const double *solver_eig(Context &ctx, const SolveGeom &g);

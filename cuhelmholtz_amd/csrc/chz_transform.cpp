@@ -55,6 +55,7 @@ int fill_pass(PassArgs &p, const AxisCfg &c, int prec, int kind, const void *tw)
     p.plan = c.plan;
     p.tw = tw;
     p.scale = 1.0;
+    p.xcds = 8;   // MI355X: 8 XCDs
     (void)prec;
     return OK;
 }
@@ -99,7 +100,8 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
     Context &ctx = Context::get();
     std::lock_guard<std::mutex> g(ctx.mutex());
     AxisCfg cx, cy, cz;
-    if (!ctx.config(NX, prec, true, &cx) || !ctx.config(NY, prec, false, &cy) || !ctx.config(NZ, prec, false, &cz))
+    if (!ctx.config(NX, prec, ROLE_ROWS, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) ||
+        !ctx.config(NZ, prec, ROLE_ZCOLS, &cz))
         return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
     const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
     if (!twx || !twy || !twz) return fail(ERR_ALLOC, "twiddle table allocation failed");
@@ -193,17 +195,43 @@ const char *chz_last_error(void) { return chz::last_error(); }
 
 void chz_release(void) { chz::Context::get().release(); }
 
-int chz_axis_config(int N, int prec, int rows, int *ept, int *tpl, int *lines, unsigned long long *plan)
+int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines, unsigned long long *plan)
 {
     chz::Context &ctx = chz::Context::get();
     std::lock_guard<std::mutex> g(ctx.mutex());
     chz::AxisCfg c;
-    if (!ctx.config(N, prec, rows != 0, &c)) return chz::fail(chz::ERR_UNSUPPORTED, "unsupported cell count");
+    if (role < 0 || role >= chz::ROLE_COUNT) return chz::fail(chz::ERR_INVALID, "invalid role");
+    if (!ctx.config(N, prec, role, &c)) return chz::fail(chz::ERR_UNSUPPORTED, "unsupported cell count");
     if (ept) *ept = c.EPT;
     if (tpl) *tpl = c.TPL;
     if (lines) *lines = c.L;
     if (plan) *plan = c.plan;
     return chz::OK;
+}
+
+int chz_set_axis_config(int N, int prec, int role, int ept, int tpl, int lines)
+{
+    chz::Context &ctx = chz::Context::get();
+    std::lock_guard<std::mutex> g(ctx.mutex());
+    if (!ctx.set_config(N, prec, role, ept, tpl, lines, nullptr))
+        return chz::fail(chz::ERR_INVALID, "launch configuration cannot run (radix plan, threads or LDS)");
+    return chz::OK;
+}
+
+void chz_reset_axis_configs(void)
+{
+    chz::Context &ctx = chz::Context::get();
+    std::lock_guard<std::mutex> g(ctx.mutex());
+    ctx.reset_configs();
+}
+
+int chz_load_tuning(const char *path)
+{
+    chz::Context &ctx = chz::Context::get();
+    std::lock_guard<std::mutex> g(ctx.mutex());
+    int n = ctx.load_tuning(path);
+    if (n < 0) return chz::fail(chz::ERR_INVALID, std::string("cannot read tuning table ") + path);
+    return n;
 }
 
 int chz_version(void) { return 100; }

@@ -64,11 +64,26 @@ const char *chz_last_error(void);
  * Replaces freeMemory_cubic (reference cudasymmfft/funcinterface.cu:337-342). */
 void chz_release(void);
 
-/* Launch configuration chosen for an axis of N cells: elements per thread,
+/* Launch configuration of a pass kernel for N cells: elements per thread,
  * threads per line, lines per workgroup and the radix plan (one byte per
- * stage).  prec = 64 or 32; rows = 1 for the contiguous (x) axis. */
-int chz_axis_config(int N, int prec, int rows, int *ept, int *tpl, int *lines,
+ * stage).  prec = 64 or 32; role 0 = x-axis rows, 1 = y columns (and the
+ * solve's y passes), 2 = first solve pass (folds + forward x), 3 = fused z
+ * pass of the solve, 4 = z columns of do_transform.  No reference
+ * counterpart (the reference's cuFFT plans are opaque). */
+int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines,
                     unsigned long long *plan);
+
+/* Pin a launch configuration (used by the autotuner); -1 if it cannot run
+ * (no radix plan for ept/tpl, more than 256 threads, or LDS over the limit). */
+int chz_set_axis_config(int N, int prec, int role, int ept, int tpl, int lines);
+
+/* Forget pinned / cached configurations (back to the built-in choice). */
+void chz_reset_axis_configs(void);
+
+/* Pin every configuration of a tuning table (lines "role N prec ept tpl
+ * lines"); returns how many were valid.  At start-up the library loads
+ * $CHZ_TUNE_FILE or tuning/<gfx arch>.txt next to the shared library. */
+int chz_load_tuning(const char *path);
 
 int chz_version(void);
 

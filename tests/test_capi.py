@@ -72,4 +72,4 @@ def test_axis_planner_covers_235_smooth_sizes():
                     nb = -(-(M // R) // cfg["tpl"])
                     assert nb <= cfg["ept"] // R, (N, cfg)
                 assert prod == M, (N, cfg)
-                assert M <= cfg["tpl"] * cfg["ept"] and cfg["tpl"] * cfg["lines"] <= 256, (N, cfg)
+                assert M <= cfg["tpl"] * cfg["ept"] and cfg["tpl"] * cfg["lines"] <= 512, (N, cfg)
