@@ -7,8 +7,13 @@ modes + 1e-3 noise (seed 1), zero Neumann data.  One step = one complete
 solve (all passes) of that input with the data resident in HBM; steps solve
 in place repeatedly (the work is data-independent).
 
+Two paths are measured: the plain FP64 solve (80 bytes/cell) and the FP32
+transforms + FP64 iterative refinement with the fewest refinement steps that
+reach 1e-10 relative error against the FP64 solve ((48 + 64 k) bytes/cell).
+
 Prints ONE JSON line (rank 0):
-  value       whole-job solves/s of the measured path
+  value       whole-job solves/s of the faster path meeting 1e-10 relative
+              error against the FP64 solve (both paths are in config)
   roofline    the dominant kernel: algorithmic bytes per launch (16 bytes per
               cell: one FP64 read + one FP64 write of the 512^3 volume) over
               its average duration, measured with HIP events recorded around
@@ -105,34 +110,66 @@ def main():
     torch.cuda.empty_cache()
     log(f"fp64 solve: relative residual {rres:.3e}")
 
-    # timed steps
-    ft = f0.clone()
-    for _ in range(args.warmup):
-        C.helmholtz_solve(ft, n, bc, lam, bct)
-    torch.cuda.synchronize()
-    C.set_timing(True)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        C.helmholtz_solve(ft, n, bc, lam, bct)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
-    passes = C.get_timing(64 * args.steps)
-    C.set_timing(False)
-    per = {}
-    for name, t in passes:
-        per.setdefault(name, []).append(t)
-    avg = {k: float(np.mean(v)) for k, v in per.items()}
-    for k, v in avg.items():
-        log(f"  pass {k:8s} {v:8.3f} ms  {16.0 * V / v / 1e6:8.0f} GB/s (16 B/cell model)")
+    def timed(solve_fn, label):
+        ft = f0.clone()
+        for _ in range(args.warmup):
+            solve_fn(ft)
+        torch.cuda.synchronize()
+        C.set_timing(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            solve_fn(ft)
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / args.steps
+        passes = C.get_timing(64 * args.steps)
+        C.set_timing(False)
+        del ft
+        per = {}
+        for name, t in passes:
+            per.setdefault(name, []).append(t)
+        # passes that run k times per solve (refinement) keep their per-launch average
+        avg = {k: float(np.mean(v)) for k, v in per.items()}
+        for k, v in avg.items():
+            log(f"  {label} pass {k:8s} {v:8.3f} ms x{len(per[k]) // args.steps}")
+        return ms, avg, wall
+
+    # plain FP64
+    ms, avg, wall = timed(lambda t: C.helmholtz_solve(t, n, bc, lam, bct), "fp64")
     dom = max((k for k in avg if k != "periodic"), key=lambda k: avg[k])
-    achieved = 16.0 * V / avg[dom] / 1e6   # GB/s
+    achieved = 16.0 * V / avg[dom] / 1e6   # GB/s: one FP64 read + write per cell
     solves = 1000.0 / ms
     log(f"fp64 solve {ms:.3f} ms -> {solves:.1f} solves/s, {80.0 * V / ms / 1e6:.0f} GB/s model (80 B/cell); wall {wall:.2f} s")
+
+    # FP32 + FP64 refinement: fewest steps reaching 1e-10 against the FP64 solve
+    ref = f0.clone()
+    C.helmholtz_solve(ref, n, bc, lam, bct)
+    mixed = None
+    for k in range(0, 5):
+        t = f0.clone()
+        C.helmholtz_solve_mixed(t, n, bc, lam, bct, refine=k)
+        err = ((t - ref).abs().max() / ref.abs().max()).item()
+        log(f"fp32 + {k} refinement(s): relative error vs fp64 {err:.3e}")
+        del t
+        if err <= 1e-10:
+            mixed = (k, err)
+            break
+    del ref
+    torch.cuda.empty_cache()
+    mixed_info = None
+    if mixed is not None:
+        k, err = mixed
+        mms, mavg, _ = timed(lambda t: C.helmholtz_solve_mixed(t, n, bc, lam, bct, refine=k), "mixed")
+        mbytes = (48.0 + 64.0 * k) * V
+        mixed_info = {"refine": k, "rel_err_vs_fp64": err, "ms": mms, "solves_per_s": 1000.0 / mms,
+                      "hbm_gbs_model": mbytes / mms / 1e6, "bytes_per_cell": 48 + 64 * k,
+                      "vs_fp64": (1000.0 / mms) / solves, "pass_ms": mavg}
+        log(f"mixed solve (k={k}) {mms:.3f} ms -> {1000 / mms:.1f} solves/s ({mixed_info['vs_fp64']:.2f}x fp64), "
+            f"{mbytes / mms / 1e6:.0f} GB/s model")
 
     # the 3D transform at the same size, for reference
     x = torch.zeros((n[2] + 2, n[1] + 2, n[0] + 2), dtype=torch.float64, device=dev)
@@ -140,6 +177,7 @@ def main():
     kinds = (C.DCT_1, C.DCT_3, C.DFT_R2C)
     C.do_transform(x, y, kinds)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(5):
         C.do_transform(x, y, kinds)
@@ -154,24 +192,28 @@ def main():
         cpu = cpu_baseline(n, bc, lam)
         log(f"cpu baseline: {cpu}")
 
+    best_path, value, best_ms = "fp64", solves, ms
+    if mixed_info and mixed_info["solves_per_s"] > solves:
+        best_path, value, best_ms = "fp32+refine", mixed_info["solves_per_s"], mixed_info["ms"]
     out = {
         "metric": metric_name(),
-        "value": solves,
+        "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": best_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"Helmholtz solve {n[0]}x{n[1]}x{n[2]}, codes (3,4,0), lambda -4, smooth modes + noise",
-                   "path": "fp64", "relative_residual": rres,
-                   "hbm_gbs_model": 80.0 * V / ms / 1e6,
-                   "transforms_per_s_fp64": 1000.0 / tms,
-                   "pass_ms": avg},
+                   "path": best_path,
+                   "fp64": {"solves_per_s": solves, "ms": ms, "relative_residual": rres,
+                            "hbm_gbs_model": 80.0 * V / ms / 1e6, "pass_ms": avg},
+                   "fp32_refine": mixed_info,
+                   "transforms_per_s_fp64": 1000.0 / tms},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,

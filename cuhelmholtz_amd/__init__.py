@@ -60,7 +60,7 @@ def _declare_solver(L):
     if hasattr(L, "chz_helmholtz_solve"):
         L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve_mixed"):
-        L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, d, vp, vp, vp]
+        L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
 
 
 def header_symbols():
@@ -172,6 +172,29 @@ def get_timing(max_passes=4096):
 
 ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS = range(5)
 ROLE_NAMES = ["rows", "cols", "xfwd", "zmid", "zcols"]
+
+
+def helmholtz_solve_mixed(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3, refine=1):
+    """In-place solve with FP32 transforms / divide and `refine` FP64 iterative
+    refinement steps (same arguments as helmholtz_solve)."""
+    import torch
+    NX, NY, NZ = (int(v) for v in n)
+    if not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous():
+        raise ValueError("f must be a contiguous CUDA float64 tensor")
+    if tuple(f.shape) != (NZ + 1, NY + 1, NX + 1):
+        raise ValueError("f must be shaped (NZ+1, NY+1, NX+1)")
+    faces, keep = _faces(bcs, (NX, NY, NZ), bc, f.device)
+    args = []
+    for a, N in enumerate((NX, NY, NZ)):
+        args += [float(dom[2 * a]), float(dom[2 * a + 1]), N, int(bc[a]), faces[2 * a], faces[2 * a + 1]]
+    _check(lib().chz_helmholtz_solve_mixed(*args, float(lam), ctypes.c_void_p(f.data_ptr()), int(refine), _stream(f)))
+    del keep
+    return f
+
+
+def helmholtz_solve_mixed_fp32only(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
+    """FP32 solve without refinement (the building block the autotuner times)."""
+    return helmholtz_solve_mixed(f, n, bc, lam, bcs, dom, refine=0)
 
 
 def axis_config(N, prec=64, rows=True, role=None):

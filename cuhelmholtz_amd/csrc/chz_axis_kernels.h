@@ -149,6 +149,157 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, Solv
     store_tile<T, false>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
+// FP64 7-point residual r = f_fold - (Delta_h U + lambda U) at unknown node
+// (s, j, k): the boundary folds of f (as in fold_boundaries) and the
+// homogeneous operator (Dirichlet neighbours 0, Neumann ghost u_{-1} = u_1,
+// periodic wrap), the discrete system of cuda_helmholtz_solver.
+// U == nullptr: the folded rhs alone.  In the singular case (sa.pertrb set)
+// the constant PERTRB (weighted mean of the folded rhs, the part of f outside
+// the range of the operator) is removed, so the correction solves of a
+// refinement only see a consistent rhs.
+__device__ __forceinline__ double residual_at(const SolveArgs &sa, const double *__restrict__ U, int s, int j, int k)
+{
+    const long long fx = sa.NX + 1, fy = sa.NY + 1, fxy = fx * fy;
+    const long long c = s + j * fx + k * fxy;
+    double acc = sa.f[c];
+    // folds, reference order x, y, z
+    const double *row = sa.f + j * fx + k * fxy;
+    if (sa.bc[0] != 0 && s == sa.s[0]) {
+        if (sa.bc[0] == 1 || sa.bc[0] == 2) acc -= row[0] * sa.ih2[0];
+        else acc += 2 * sa.bcl[k * fy + j] * sa.ih[0];
+    }
+    if (sa.bc[0] != 0 && s == sa.e[0]) {
+        if (sa.bc[0] == 1 || sa.bc[0] == 4) acc -= row[sa.NX] * sa.ih2[0];
+        else acc -= 2 * sa.bcr[k * fy + j] * sa.ih[0];
+    }
+    if (sa.bc[1] != 0 && j == sa.s[1]) {
+        if (sa.bc[1] == 1 || sa.bc[1] == 2) acc -= sa.f[s + k * fxy] * sa.ih2[1];
+        else acc += 2 * sa.bcb[k * fx + s] * sa.ih[1];
+    }
+    if (sa.bc[1] != 0 && j == sa.e[1]) {
+        if (sa.bc[1] == 1 || sa.bc[1] == 4) acc -= sa.f[s + sa.NY * fx + k * fxy] * sa.ih2[1];
+        else acc -= 2 * sa.bct[k * fx + s] * sa.ih[1];
+    }
+    if (sa.bc[2] != 0 && k == sa.s[2]) {
+        if (sa.bc[2] == 1 || sa.bc[2] == 2) acc -= sa.f[s + j * fx] * sa.ih2[2];
+        else acc += 2 * sa.bcf[j * fx + s] * sa.ih[2];
+    }
+    if (sa.bc[2] != 0 && k == sa.e[2]) {
+        if (sa.bc[2] == 1 || sa.bc[2] == 4) acc -= sa.f[s + j * fx + sa.NZ * fxy] * sa.ih2[2];
+        else acc -= 2 * sa.bce[j * fx + s] * sa.ih[2];
+    }
+    if (U == nullptr) return acc;
+    if (sa.pertrb) acc -= sa.pertrb[0] * sa.pert_scale;
+    // homogeneous operator
+    const double u0 = U[c];
+    const int q[3] = {s, j, k};
+    const long long st[3] = {1, fx, fxy};
+    double au = sa.lambda * u0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int lo = sa.s[a], hi = sa.e[a], bc = sa.bc[a];
+        double um, up;
+        if (q[a] > lo) um = U[c - st[a]];
+        else um = bc == 0 ? U[c + (long long)(hi - lo) * st[a]] : (bc == 3 || bc == 4) ? (hi > lo ? U[c + st[a]] : 0.0) : 0.0;
+        if (q[a] < hi) up = U[c + st[a]];
+        else up = bc == 0 ? U[c - (long long)(hi - lo) * st[a]] : (bc == 2 || bc == 3) ? (hi > lo ? U[c - st[a]] : 0.0) : 0.0;
+        au += (um - 2 * u0 + up) * sa.ih2[a];
+    }
+    return acc - au;
+}
+
+// Refinement pass 1: FP64 residual of the rows (j,k) of the unknown box for
+// the current solution U, rounded to T, forward x transform -> workspace.
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, SolveArgs sa, const double *U)
+{
+    typedef typename vec2_of<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = p.L;
+    const Smem<T> sm(smem, p.M, L);
+    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const int u = (int)threadIdx.x / L;
+    const V *tw = reinterpret_cast<const V *>(p.tw);
+    int a_tile, k;
+    tile_of(p, a_tile, k);
+    const int nlines = min(L, p.a0 + p.na - a_tile);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
+    {
+        constexpr int CH = 8;
+        const int t = threadIdx.x, nt = blockDim.x, NS = p.N + 2;
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+        const int total = L * NS;
+        for (int i0 = t; i0 < total; i0 += CH * nt) {
+            double v[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int idx = i0 + i * nt;
+                const int r = idx / NS, sl = idx - (idx / NS) * NS;
+                v[i] = 0.0;
+                if (idx < total && r < nlines && sl >= sa.s[0] && sl <= sa.e[0]) v[i] = residual_at(sa, U, sl, a_tile + r, k);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int idx = i0 + i * nt;
+                const int r = idx / NS, sl = idx - (idx / NS) * NS;
+                if (idx < total) rbuf[(size_t)((sl >> 1) * tl.LS + r) * 2 + (sl & 1)] = (T)v[i];
+            }
+        }
+    }
+    __syncthreads();
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
+    store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+}
+
+// Last pass of a mixed-precision solve: inverse x transform of the workspace
+// rows, then out = Uin + ratio * delta in FP64 over the unknown x window
+// (Uin == nullptr: out = ratio * delta).
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xinv_upd(PassArgs p, const double *Uin, double *out)
+{
+    typedef typename vec2_of<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = p.L;
+    const Smem<T> sm(smem, p.M, L);
+    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const int u = (int)threadIdx.x / L;
+    const V *tw = reinterpret_cast<const V *>(p.tw);
+    int a_tile, k;
+    tile_of(p, a_tile, k);
+    const int nlines = min(L, p.a0 + p.na - a_tile);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
+    load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src));
+    __syncthreads();
+    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
+    {
+        constexpr int CH = 16;
+        const int t = threadIdx.x, nt = blockDim.x;
+        const int RL = p.d_rowlen;
+        const int total = nlines * RL;
+        const long long base = (long long)a_tile * p.d_sa + (long long)k * p.d_sb;
+        const T *rbuf = reinterpret_cast<const T *>(tl.buf);
+        const double sc = p.scale;
+        for (int i0 = t; i0 < total; i0 += CH * nt) {
+            double v[CH];
+            bool w[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int idx = i0 + i * nt;
+                const int r = idx / RL, sl = idx - (idx / RL) * RL;
+                w[i] = idx < total && sl >= p.st_lo && sl <= p.st_hi;
+                v[i] = 0.0;
+                if (w[i]) {
+                    v[i] = (double)rbuf[(size_t)((sl >> 1) * tl.LS + r) * 2 + (sl & 1)] * sc;
+                    if (Uin) v[i] += Uin[base + idx];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++)
+                if (w[i]) out[base + i0 + i * nt] = v[i];
+        }
+    }
+}
+
 // Periodic duplicate node N <- node 0 along one axis over the whole face
 // (cudahelmholtz/_cuda_helmholtz_.inl:58-68, 116-126, 173-183).
 template <typename T>

@@ -55,6 +55,8 @@ struct SolveArgs {
     double lambda;
     int singular;                                      // zero the (0,0,0) mode
     int kind2, out2_lo, out2_hi;                       // inverse kind of the middle pass and its window
+    const double *pertrb;                              // singular case: weighted sum of the folded rhs (or null)
+    double pert_scale;                                 // 1 / (sum of the 0-mode weights)
 };
 
 __host__ __device__ inline int lds_points(int M, int L) { return (M + 1) * (L + 1); }
@@ -68,12 +70,24 @@ const void *solve_xfwd_kernel_f64(int ept);
 const void *solve_xfwd_kernel_f32(int ept);
 const void *solve_zmid_kernel_f64(int ept);
 const void *solve_zmid_kernel_f32(int ept);
+const void *solve_xres_kernel_f32(int ept);
+const void *solve_xinv_upd_kernel_f32(int ept);
 const void *periodic_copy_kernel();
 
 // Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
 // FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.
 hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s);
 hipError_t launch_solve_pass(const void *fn, const PassArgs &p, const SolveArgs &sa, int prec, int nb, hipStream_t s);
+hipError_t launch_pass_ptrs(const void *fn, const PassArgs &p, const void *a0, const void *a1, int prec, int nb,
+                            hipStream_t s);
+hipError_t launch_pass_sa_ptr(const void *fn, const PassArgs &p, const SolveArgs &sa, const void *a0, int prec, int nb,
+                              hipStream_t s);
 size_t axis_pass_lds(int prec, int M, int L, int TPL);
+// 1 / (sum of the 0-mode weights) over the unknown box
+double mode0_inv_wsum(const SolveArgs &sa);
+// subtract the weighted mean (the (0,0,0) mode) of u over the unknown box; acc: one device double
+hipError_t launch_remove_mode0(double *u, const SolveArgs &sa, double *acc, hipStream_t s);
+// acc = weighted sum of the folded rhs over the unknown box (PERTRB x weight sum)
+hipError_t launch_fold_sum(const SolveArgs &sa, double *acc, hipStream_t s);
 
 }  // namespace chz

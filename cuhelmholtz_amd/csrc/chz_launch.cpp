@@ -31,4 +31,27 @@ hipError_t launch_solve_pass(const void *fn, const PassArgs &p, const SolveArgs 
     return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
 }
 
+hipError_t launch_pass_ptrs(const void *fn, const PassArgs &p, const void *a0, const void *a1, int prec, int nb,
+                            hipStream_t s)
+{
+    if (!fn) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
+    PassArgs a = p;
+    const void *x = a0, *y = a1;
+    void *args[] = {&a, &x, &y};
+    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
+}
+
+hipError_t launch_pass_sa_ptr(const void *fn, const PassArgs &p, const SolveArgs &sa, const void *a0, int prec, int nb,
+                              hipStream_t s)
+{
+    if (!fn) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
+    PassArgs a = p;
+    SolveArgs b = sa;
+    const void *x = a0;
+    void *args[] = {&a, &b, &x};
+    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
+}
+
 }  // namespace chz

@@ -123,10 +123,12 @@ void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const d
     sa.singular = g.singular ? 1 : 0;
 }
 
-// The 5-pass spectral core on workspace ex (padded, precision of T) with the
-// rhs read from f (node layout, FP64) and the solution written back to f.
+// Passes 1-4 of a solve on workspace ex (precision of T, rows padded to
+// 128 bytes): pass 1 reads f (node layout, FP64) and applies the boundary
+// folds, or -- with U set -- forms the FP64 residual f_fold - A U (a
+// refinement step of the mixed-precision solve).
 template <typename T>
-int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st)
+int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st, const double *U, const double *pertrb)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
@@ -145,6 +147,8 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
     out_window(g.fwd[2], NZ, &szl, &szh);
     SolveArgs sa;
     fill_solve_args(sa, g, f, eig);
+    sa.pertrb = pertrb;
+    sa.pert_scale = pertrb ? mode0_inv_wsum(sa) : 0.0;
     int rc;
     hipError_t e;
     const bool tm = ctx.timing();
@@ -162,10 +166,17 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
         p.out_lo = sxl; p.out_hi = sxh;
         const int nb = g.e[2] - g.s[2] + 1;
         if (!pass_in_bounds(p, true, nb, fvol, vol)) return fail(ERR_INVALID, "pass 1 geometry outside the buffers");
-        const void *fn = prec == 64 ? solve_xfwd_kernel_f64(cx.EPT) : solve_xfwd_kernel_f32(cx.EPT);
-        if (tm) ctx.timing_mark("xfwd", st);
-        if ((e = launch_solve_pass(fn, p, sa, prec, nb, st)) != hipSuccess)
-            return fail(ERR_DEVICE, std::string("pass 1 launch: ") + hipGetErrorString(e));
+        if (U) {
+            if (prec != 32) return fail(ERR_INVALID, "residual pass is FP32 only");
+            if (tm) ctx.timing_mark("xres", st);
+            if ((e = launch_pass_sa_ptr(solve_xres_kernel_f32(cx.EPT), p, sa, U, prec, nb, st)) != hipSuccess)
+                return fail(ERR_DEVICE, std::string("residual pass launch: ") + hipGetErrorString(e));
+        } else {
+            const void *fn = prec == 64 ? solve_xfwd_kernel_f64(cx.EPT) : solve_xfwd_kernel_f32(cx.EPT);
+            if (tm) ctx.timing_mark("xfwd", st);
+            if ((e = launch_solve_pass(fn, p, sa, prec, nb, st)) != hipSuccess)
+                return fail(ERR_DEVICE, std::string("pass 1 launch: ") + hipGetErrorString(e));
+        }
     }
     // pass 2: forward y, columns x in the x spectral window, z in the unknown window
     {
@@ -276,17 +287,105 @@ int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const d
     double *ex = reinterpret_cast<double *>(ctx.workspace(bytes, 0));
     if (!ex) return fail(ERR_ALLOC, "workspace allocation failed");
     PassTimer timer(ctx, st);
-    if ((rc = solve_core<double>(ctx, g, f, ex, st))) return rc;
+    if ((rc = solve_core<double>(ctx, g, f, ex, st, nullptr, nullptr))) return rc;
     if ((rc = solve_store<double>(ctx, g, f, ex, st))) return rc;
     timer.mark("periodic");
     return periodic_copies(g, f, st);
 }
 
-template int solve_core<float>(Context &, const SolveGeom &, double *, float *, hipStream_t);
+// Last pass of the mixed-precision solve: inverse x of the FP32 workspace
+// rows, out = Uin + ratio * delta (FP64) over the unknown box.
+static int solve_update(Context &ctx, const SolveGeom &g, const float *ex, const double *Uin, double *out,
+                        hipStream_t st)
+{
+    const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
+    AxisCfg cx;
+    if (!ctx.config(NX, 32, ROLE_ROWS, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
+    const void *twx = ctx.twiddles(NX, 32);
+    const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(float)), P1 = NY + 2;
+    const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
+    int sxl, sxh;
+    out_window(g.fwd[0], NX, &sxl, &sxh);
+    PassArgs p{};
+    fill_pass(p, cx, 32, g.inv[0], twx);
+    p.src = ex; p.dst = out;
+    p.s_sa = P0; p.s_sb = P0 * P1; p.s_sp = 1; p.s_rowlen = (int)P0;
+    p.d_sa = fx; p.d_sb = fx * fy; p.d_sp = 1; p.d_rowlen = (int)fx;
+    p.a0 = g.s[1]; p.na = g.e[1] - g.s[1] + 1; p.b0 = g.s[2];
+    p.in_lo = sxl; p.in_hi = sxh;
+    p.st_lo = g.s[0]; p.st_hi = g.e[0];
+    p.out_lo = g.s[0]; p.out_hi = g.e[0];
+    p.scale = g.ratio;
+    const int nb = g.e[2] - g.s[2] + 1;
+    if (!pass_in_bounds(p, true, nb, vol, fvol)) return fail(ERR_INVALID, "update pass geometry outside the buffers");
+    if (ctx.timing()) ctx.timing_mark(Uin ? "xupd" : "xinv", st);
+    hipError_t e = launch_pass_ptrs(solve_xinv_upd_kernel_f32(cx.EPT), p, Uin, out, 32, nb, st);
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("update pass launch: ") + hipGetErrorString(e));
+    return OK;
+}
+
+// FP32 transforms and divide with FP64 iterative refinement: an FP32 solve
+// of the folded rhs, then `refine` times: FP64 residual -> FP32 correction
+// solve -> FP64 update.  Twiddles and eigenvalues are generated in FP64 and
+// rounded to FP32; the residual and the update are the only FP64 arithmetic.
+int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
+                    double *f, int refine, hipStream_t st)
+{
+    if (!f) return fail(ERR_INVALID, "null solution array");
+    if (refine < 0 || refine > 16) return fail(ERR_INVALID, "refinement count must be in 0..16");
+    SolveGeom g;
+    int rc = make_geom(g, dom, N, bc, lambda, faces);
+    if (rc) return rc;
+    Context &ctx = Context::get();
+    std::lock_guard<std::mutex> lock(ctx.mutex());
+    const size_t wbytes = sizeof(float) * (size_t)ws_row(N[0], sizeof(float)) * (N[1] + 2) * (N[2] + 2);
+    const size_t ubytes = sizeof(double) * (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
+    float *ex = reinterpret_cast<float *>(ctx.workspace(wbytes, 1));
+    double *U = refine > 0 ? reinterpret_cast<double *>(ctx.workspace(ubytes, 2)) : nullptr;
+    if (!ex || (refine > 0 && !U)) return fail(ERR_ALLOC, "workspace allocation failed");
+    PassTimer timer(ctx, st);
+    // singular case: PERTRB (weighted sum of the folded rhs) in FP64, once
+    double *acc = nullptr;
+    SolveArgs sa0;
+    fill_solve_args(sa0, g, f, nullptr);
+    if (g.singular && refine > 0) {
+        acc = reinterpret_cast<double *>(ctx.workspace(256, 3));
+        if (!acc) return fail(ERR_ALLOC, "workspace allocation failed");
+        if (ctx.timing()) ctx.timing_mark("pertrb", st);
+        hipError_t e = launch_fold_sum(sa0, acc + 1, st);
+        if (e != hipSuccess) return fail(ERR_DEVICE, std::string("PERTRB launch: ") + hipGetErrorString(e));
+    }
+    if ((rc = solve_core<float>(ctx, g, f, ex, st, nullptr, nullptr))) return rc;
+    if ((rc = solve_update(ctx, g, ex, nullptr, refine > 0 ? U : f, st))) return rc;
+    for (int it = 1; it <= refine; it++) {
+        if ((rc = solve_core<float>(ctx, g, f, ex, st, U, acc ? acc + 1 : nullptr))) return rc;
+        if ((rc = solve_update(ctx, g, ex, U, it == refine ? f : U, st))) return rc;
+    }
+    if (acc) {
+        // the corrections never change the 0-mode of the FP32 solve (rounding
+        // size): remove it so the result has the zero mode of the FP64 path
+        if (ctx.timing()) ctx.timing_mark("mode0", st);
+        hipError_t e = launch_remove_mode0(f, sa0, acc, st);
+        if (e != hipSuccess) return fail(ERR_DEVICE, std::string("mode-0 projection launch: ") + hipGetErrorString(e));
+    }
+    timer.mark("periodic");
+    return periodic_copies(g, f, st);
+}
 
 }  // namespace chz
 
 extern "C" {
+
+int chz_helmholtz_solve_mixed(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                              double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                              double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                              double lambda, double *f, int refine, void *stream)
+{
+    const double dom[6] = {left, right, bottom, top, front, back};
+    const int N[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
+    const double *faces[6] = {bcl, bcr, bcb, bct, bcf, bce};
+    return chz::helmholtz_mixed(dom, N, bc, faces, lambda, f, refine, (hipStream_t)stream);
+}
 
 int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
                         double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,

@@ -24,9 +24,11 @@ void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const d
 
 // passes 1-4 of a solve into workspace ex (rhs read from f)
 template <typename T>
-int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st);
+int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st, const double *U, const double *pertrb);
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 
+int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
+                    double *f, int refine, hipStream_t st);
 int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
                    double *f, hipStream_t st);
 

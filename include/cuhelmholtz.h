@@ -57,6 +57,19 @@ int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double
                         double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
                         double lambda, double *f, void *stream);
 
+/* Same solve with FP32 transforms and divide plus `refine` (0..16) steps of
+ * FP64 iterative refinement: an FP32 solve of the folded rhs, then per step
+ * the FP64 7-point residual r = f - (Delta_h u + lambda u), an FP32
+ * correction solve and the FP64 update u += delta.  Twiddles and eigenvalues
+ * are generated in FP64 and rounded; the residual and the update are the only
+ * FP64 arithmetic.  No reference counterpart (the reference is FP64 only);
+ * same arguments and layouts as chz_helmholtz_solve.  Workspaces:
+ * (NX+2)(NY+2)(NZ+2) floats and, if refine > 0, (NX+1)(NY+1)(NZ+1) doubles. */
+int chz_helmholtz_solve_mixed(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                              double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                              double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                              double lambda, double *f, int refine, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *chz_last_error(void);
 

@@ -155,3 +155,57 @@ def test_cpp_dropin_cuda_helmholtz_solver():
             -1.0, vp(ft.data_ptr()))
     assert rc == 0
     assert rel(ft.cpu().numpy(), want) < TOL
+
+
+# ---------------------------------------------------------------------------
+# FP32 transforms + FP64 iterative refinement, against the plain FP64 solve
+# ---------------------------------------------------------------------------
+def mixed_solve(f, n, bc, lam, bcs, refine, dom=(0.0, 1.0) * 3):
+    ft = torch.from_numpy(np.ascontiguousarray(f)).to(dev())
+    C.helmholtz_solve_mixed(ft, n, bc, lam, bcs, dom, refine=refine)
+    torch.cuda.synchronize()
+    return ft.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [(6, 8, 10), (16, 12, 8)])
+def test_mixed_all_bc_converges(n):
+    """Error vs the FP64 solve: FP32 alone ~1e-6, one refinement < 1e-10, two < 1e-12."""
+    rng = np.random.default_rng(11)
+    for bc in itertools.product(range(5), repeat=3):
+        f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+        bcs = faces(rng, n, bc)
+        ref = gpu_solve(f, n, bc, -2.3, bcs)
+        e0 = rel(mixed_solve(f, n, bc, -2.3, bcs, 0), ref)
+        e1 = rel(mixed_solve(f, n, bc, -2.3, bcs, 1), ref)
+        e2 = rel(mixed_solve(f, n, bc, -2.3, bcs, 2), ref)
+        assert e0 < 1e-4 and e1 < 1e-10 and e2 < 1e-12, (bc, e0, e1, e2)
+        assert e1 < e0
+
+
+def test_mixed_singular_and_periodic():
+    rng = np.random.default_rng(12)
+    n = (8, 12, 6)
+    for bc in [(0, 0, 0), (3, 3, 3), (0, 3, 0)]:
+        f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+        bcs = faces(rng, n, bc)
+        ref = gpu_solve(f, n, bc, 0.0, bcs)
+        assert rel(mixed_solve(f, n, bc, 0.0, bcs, 2), ref) < 1e-12, bc
+
+
+@pytest.mark.parametrize("n", [(512, 384, 256), (512, 512, 512)])
+@pytest.mark.parametrize("lam", [-4.0, 0.5])
+def test_mixed_full_size_1e10(n, lam):
+    """Mixed-precision workload: FP32 + refinement reaches 1e-10 relative error
+    against the plain FP64 solve within two refinement steps."""
+    f, bcs = W.smooth_modes(n)
+    ref = torch.from_numpy(f).to(dev())
+    C.helmholtz_solve(ref, n, (3, 4, 0), lam, bcs)
+    errs = []
+    for refine in (1, 2):
+        ft = torch.from_numpy(f).to(dev())
+        C.helmholtz_solve_mixed(ft, n, (3, 4, 0), lam, bcs, refine=refine)
+        errs.append(((ft - ref).abs().max() / ref.abs().max()).item())
+        del ft
+    assert min(errs) < 1e-10, errs
+    del ref
+    torch.cuda.empty_cache()
