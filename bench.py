@@ -47,6 +47,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# kernel behind each FP64 solve pass (for the HBM traffic measured by rocprofv3)
+PASS_KERNEL = {"xfwd": "chz::k_solve_xfwd<double, 8>", "yfwd": "chz::k_axis_pass<double, double, double, 8, false>",
+               "zmid": "chz::k_solve_zmid<double, 8>", "yinv": "chz::k_axis_pass<double, double, double, 8, false>",
+               "xinv": "chz::k_axis_pass<double, double, double, 8, true>"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from profiles/traffic_512.txt (two
+    rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, calibrated against a copy
+    kernel of known size); None if not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic_512.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            parts = line.rsplit(None, 2)
+            if parts[0].strip() == kernel:
+                return (float(parts[1]) + float(parts[2])) * 1e9
+    return None
+
+
 def cpu_baseline(n, bc, lam):
     """hw3crt-contract CPU solver on a 512x512x64 slab, scaled to a full solve."""
     import numpy as np
@@ -214,8 +237,9 @@ def main():
                             "hbm_gbs_model": 80.0 * V / ms / 1e6, "pass_ms": avg},
                    "fp32_refine": mixed_info,
                    "transforms_per_s_fp64": 1000.0 / tms},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": PASS_KERNEL.get(dom, dom), "pass": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(PASS_KERNEL.get(dom, dom)) if n == (512, 512, 512) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
