@@ -52,6 +52,7 @@ def lib():
                                       ctypes.POINTER(ctypes.c_ulonglong)]
         _lib = L
         _declare_solver(L)
+        _declare_batch(L)
     return _lib
 
 
@@ -61,6 +62,20 @@ def _declare_solver(L):
         L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve_mixed"):
         L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("domain", ctypes.c_double * 6), ("N", ctypes.c_int * 3), ("bc", ctypes.c_int * 3),
+                ("lam", ctypes.c_double), ("f", ctypes.c_void_p), ("faces", ctypes.c_void_p * 6)]
+
+
+def _declare_batch(L):
+    vp, i = ctypes.c_void_p, ctypes.c_int
+    L.chz_batch_create.argtypes = [ctypes.POINTER(_Problem), i, i, i, ctypes.POINTER(vp)]
+    L.chz_batch_solve.argtypes = [vp, i, vp]
+    L.chz_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_longlong)]
+    L.chz_batch_destroy.argtypes = [vp]
 
 
 def header_symbols():
@@ -168,6 +183,80 @@ def get_timing(max_passes=4096):
     if n < 0:
         _check(n)
     return list(zip(names.value.decode().split(",")[:n], [ms[i] for i in range(n)]))
+
+
+class BatchSolver:
+    """Batched solve of many independent problems (chz_batch_create): one
+    hipGraph per plan, one persistent launch per pass and size class.
+
+    problems: sequence of (f, n, bc, lam[, bcs[, dom]]) with f a contiguous
+    CUDA float64 node tensor shaped (NZ+1, NY+1, NX+1) -- bound to the plan:
+    solve() solves whatever the tensors hold, in place.  prec 64 = FP64,
+    prec 32 = FP32 transforms + `refine` FP64 refinement steps.
+    """
+
+    def __init__(self, problems, prec=64, refine=0):
+        import torch
+        self._keep = []
+        arr = (_Problem * len(problems))()
+        dev = None
+        for k, pr in enumerate(problems):
+            f, n, bc, lam = pr[:4]
+            bcs = pr[4] if len(pr) > 4 and pr[4] is not None else (None,) * 6
+            dom = pr[5] if len(pr) > 5 else (0.0, 1.0) * 3
+            NX, NY, NZ = (int(v) for v in n)
+            if not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous():
+                raise ValueError(f"problem {k}: f must be a contiguous CUDA float64 tensor")
+            if tuple(f.shape) != (NZ + 1, NY + 1, NX + 1):
+                raise ValueError(f"problem {k}: f must be shaped (NZ+1, NY+1, NX+1)")
+            if dev is None:
+                dev = f.device
+            elif f.device != dev:
+                raise ValueError("all problems must live on one device")
+            faces, keep = _faces(bcs, (NX, NY, NZ), bc, f.device)
+            self._keep += [f] + keep
+            q = arr[k]
+            for i in range(6):
+                q.domain[i] = float(dom[i])
+                q.faces[i] = faces[i].value if faces[i] is not None else None
+            for a, v in enumerate((NX, NY, NZ)):
+                q.N[a] = v
+                q.bc[a] = int(bc[a])
+            q.lam = float(lam)
+            q.f = f.data_ptr()
+        self.device = dev
+        self.count = len(problems)
+        self.prec, self.refine = prec, refine
+        self._plan = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            _check(lib().chz_batch_create(arr, len(problems), int(prec), int(refine), ctypes.byref(self._plan)))
+
+    def solve(self, graph=True):
+        """Run the batched solve on the current stream (graph=False: direct
+        launches, timed per launch when set_timing is on)."""
+        import torch
+        with torch.cuda.device(self.device):
+            s = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+            _check(lib().chz_batch_solve(self._plan, 1 if graph else 0, s))
+
+    def info(self):
+        n, w, t = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().chz_batch_info(self._plan, ctypes.byref(n), ctypes.byref(w), ctypes.byref(t)))
+        return {"launches": n.value, "work_bytes": w.value, "table_bytes": t.value}
+
+    def close(self):
+        if self._plan:
+            import torch
+            torch.cuda.synchronize(self.device)
+            lib().chz_batch_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS = range(5)

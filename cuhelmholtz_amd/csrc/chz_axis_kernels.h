@@ -8,24 +8,29 @@ namespace chz {
 
 // One fused axis pass over a tile of L lines: load -> fold -> FFT -> unfold -> store.
 template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
+__device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long long b, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    int a_tile, bi;
-    tile_of(p, a_tile, bi);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    const long long b = bi;
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     store_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
+}
+
+template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int a_tile, b;
+    tile_of(p, a_tile, b);
+    body_axis<T, SRC, DST, EPT, ROWS>(p, a_tile, b, smem);
 }
 
 // Boundary folding of the reference (cudahelmholtz/_cuda_helmholtz_.inl:70-227),
@@ -84,17 +89,14 @@ __device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<
 // Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
 // folds, forward x transform, store into the padded spectral workspace.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, SolveArgs sa)
+__device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa, int a_tile, int k, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    int a_tile, k;
-    tile_of(p, a_tile, k);
     const int nlines = min(L, p.a0 + p.na - a_tile);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
@@ -105,22 +107,28 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, Solv
     store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, SolveArgs sa)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int a_tile, k;
+    tile_of(p, a_tile, k);
+    body_xfwd<T, EPT>(p, sa, a_tile, k, smem);
+}
+
 // Solve pass 3: z lines (x, y) of the spectral box: forward z transform,
 // multiply by 1/(eig_x + eig_y + eig_z + lambda) (the reference's
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
 // inverse z transform, all in LDS.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, SolveArgs sa)
+__device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa, int a_tile, int y, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int L = p.L;
     const int c = (int)threadIdx.x % L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, c};
     const int u = (int)threadIdx.x / L;
-    int a_tile, y;
-    tile_of(p, a_tile, y);
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const V *tw = reinterpret_cast<const V *>(p.tw);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
@@ -147,6 +155,15 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, Solv
     q.out_lo = sa.out2_lo;
     q.out_hi = sa.out2_hi;
     store_tile<T, false>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
+}
+
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, SolveArgs sa)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int a_tile, y;
+    tile_of(p, a_tile, y);
+    body_zmid<T, EPT>(p, sa, a_tile, y, smem);
 }
 
 // FP64 7-point residual r = f_fold - (Delta_h U + lambda U) at unknown node
@@ -211,17 +228,15 @@ __device__ __forceinline__ double residual_at(const SolveArgs &sa, const double 
 // Refinement pass 1: FP64 residual of the rows (j,k) of the unknown box for
 // the current solution U, rounded to T, forward x transform -> workspace.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, SolveArgs sa, const double *U)
+__device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa, const double *U, int a_tile, int k,
+                                          unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    int a_tile, k;
-    tile_of(p, a_tile, k);
     const int nlines = min(L, p.a0 + p.na - a_tile);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     {
@@ -251,21 +266,28 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, Solv
     store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, SolveArgs sa, const double *U)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int a_tile, k;
+    tile_of(p, a_tile, k);
+    body_xres<T, EPT>(p, sa, U, a_tile, k, smem);
+}
+
 // Last pass of a mixed-precision solve: inverse x transform of the workspace
 // rows, then out = Uin + ratio * delta in FP64 over the unknown x window
 // (Uin == nullptr: out = ratio * delta).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xinv_upd(PassArgs p, const double *Uin, double *out)
+__device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, double *out, int a_tile, int k,
+                                          unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    int a_tile, k;
-    tile_of(p, a_tile, k);
     const int nlines = min(L, p.a0 + p.na - a_tile);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
     load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src));
@@ -300,6 +322,15 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xinv_upd(PassArgs p, 
     }
 }
 
+template <typename T, int EPT>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xinv_upd(PassArgs p, const double *Uin, double *out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int a_tile, k;
+    tile_of(p, a_tile, k);
+    body_xupd<T, EPT>(p, Uin, out, a_tile, k, smem);
+}
+
 // Periodic duplicate node N <- node 0 along one axis over the whole face
 // (cudahelmholtz/_cuda_helmholtz_.inl:58-68, 116-126, 173-183).
 template <typename T>
@@ -314,6 +345,66 @@ __global__ void __launch_bounds__(256) k_periodic_copy(T *f, int NX, int NY, int
     if (axis == 0) f[NX + a * fx + b * fx * fy] = f[a * fx + b * fx * fy];
     else if (axis == 1) f[a + NY * fx + b * fx * fy] = f[a + b * fx * fy];
     else f[a + b * fx + NZ * fx * fy] = f[a + b * fx];
+}
+
+// ---------------------------------------------------------------------------
+// Batched passes.  A batched solve runs, per pass of the solve, one launch per
+// size class (problems sharing the cell count along the pass axis, hence the
+// launch configuration).  Each launch is persistent: its workgroups walk the
+// class's list of work items (problem, tile).  The items are cut into 8
+// contiguous chunks, one per XCD (workgroup b runs on XCD b % 8), so the
+// tiles of one problem share an L2.  The pass arguments of every problem come
+// from a descriptor table.
+// ---------------------------------------------------------------------------
+template <typename T, int EPT, int OP>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_batch(const BatchItem *__restrict__ items, int nitems,
+                                                           const BatchDesc *__restrict__ descs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int G = (int)gridDim.x, bid = (int)blockIdx.x;
+    int w, step, end;
+    if (G >= 8 && G % 8 == 0) {
+        const int per = (nitems + 7) / 8, x = bid % 8;
+        w = x * per + bid / 8;
+        step = G / 8;
+        end = min(nitems, (x + 1) * per);
+    } else {
+        w = bid;
+        step = G;
+        end = nitems;
+    }
+    for (; w < end; w += step) {
+        const BatchItem it = items[w];
+        const BatchDesc *d = descs + it.prob;
+        const PassArgs p = d->p;
+        const int ntiles = (p.na + p.L - 1) / p.L;
+        const int a_tile = p.a0 + (it.tile % ntiles) * p.L, b = p.b0 + it.tile / ntiles;
+        if constexpr (OP == B_ROWS) body_axis<T, T, T, EPT, true>(p, a_tile, b, smem);
+        if constexpr (OP == B_COLS) body_axis<T, T, T, EPT, false>(p, a_tile, b, smem);
+        if constexpr (OP == B_XFWD) body_xfwd<T, EPT>(p, d->sa, a_tile, b, smem);
+        if constexpr (OP == B_ZMID) body_zmid<T, EPT>(p, d->sa, a_tile, b, smem);
+        if constexpr (OP == B_XRES) body_xres<T, EPT>(p, d->sa, d->u_in, a_tile, b, smem);
+        if constexpr (OP == B_XUPD) body_xupd<T, EPT>(p, d->u_in, d->u_out, a_tile, b, smem);
+        __syncthreads();   // the next item reuses the tile memory
+    }
+}
+
+// periodic duplicate copies of many problems: copy blockIdx.y, face element x
+template <typename C>
+__global__ void __launch_bounds__(256) k_batch_periodic(const C *__restrict__ tab)
+{
+    const C c = tab[blockIdx.y];
+    const long long fx = c.NX + 1, fy = c.NY + 1;
+    const int n1 = c.axis == 0 ? c.NY + 1 : c.NX + 1;
+    const int n2 = c.axis == 2 ? c.NY + 1 : c.NZ + 1;
+    double *f = c.f;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < (long long)n1 * n2;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(idx % n1), b = (int)(idx / n1);
+        if (c.axis == 0) f[c.NX + a * fx + b * fx * fy] = f[a * fx + b * fx * fy];
+        else if (c.axis == 1) f[a + c.NY * fx + b * fx * fy] = f[a + b * fx * fy];
+        else f[a + b * fx + c.NZ * fx * fy] = f[a + b * fx];
+    }
 }
 
 #define CHZ_EPT_SWITCH(ept, EXPR_TEMPLATE)  \

@@ -61,6 +61,28 @@ struct SolveArgs {
 
 __host__ __device__ inline int lds_points(int M, int L) { return (M + 1) * (L + 1); }
 
+// Pass kernels of a solve (also the op of a batched pass).
+enum BatchOp { B_ROWS = 0, B_COLS = 1, B_XFWD = 2, B_ZMID = 3, B_XRES = 4, B_XUPD = 5 };
+
+// Descriptor of one pass of one problem of a batched solve.
+struct BatchDesc {
+    PassArgs p;
+    SolveArgs sa;          // folds / eigenvalues (passes 1 and 3)
+    const double *u_in;    // residual pass: current solution; update pass: solution added to (or null)
+    double *u_out;         // update pass: destination
+};
+// one work item of a batched pass: a tile of one problem (index into the descriptors of the launch)
+struct BatchItem {
+    int prob, tile;
+};
+// one periodic face copy of a batched solve
+struct BatchCopy {
+    double *f;
+    int NX, NY, NZ, axis;
+};
+
+
+
 // kernel getters (one translation unit per precision)
 const void *axis_kernel_f64_rows(int ept);
 const void *axis_kernel_f64_cols(int ept);
@@ -73,6 +95,11 @@ const void *solve_zmid_kernel_f32(int ept);
 const void *solve_xres_kernel_f32(int ept);
 const void *solve_xinv_upd_kernel_f32(int ept);
 const void *periodic_copy_kernel();
+// batched pass kernels: op 0 rows, 1 columns, 2 solve pass 1, 3 fused z pass,
+// 4 residual pass (FP32), 5 update pass (FP32); nullptr where not built
+const void *batch_kernel_f64(int op, int ept);
+const void *batch_kernel_f32(int op, int ept);
+const void *batch_periodic_kernel();
 
 // Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
 // FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.

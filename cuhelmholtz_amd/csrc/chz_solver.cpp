@@ -123,12 +123,20 @@ void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const d
     sa.singular = g.singular ? 1 : 0;
 }
 
+static int check_spec(const PassSpec &q, long long svol, long long dvol, const char *what)
+{
+    const bool rows = q.op == B_ROWS || q.op == B_XFWD || q.op == B_XRES || q.op == B_XUPD;
+    if (!pass_in_bounds(q.d.p, rows, q.nb, svol, dvol)) return fail(ERR_INVALID, std::string(what) + ": pass geometry outside the buffers");
+    return OK;
+}
+
 // Passes 1-4 of a solve on workspace ex (precision of T, rows padded to
 // 128 bytes): pass 1 reads f (node layout, FP64) and applies the boundary
 // folds, or -- with U set -- forms the FP64 residual f_fold - A U (a
 // refinement step of the mixed-precision solve).
 template <typename T>
-int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st, const double *U, const double *pertrb)
+int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
+                 std::vector<PassSpec> &out)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
@@ -138,6 +146,7 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
     const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
     const double *eig = solver_eig(ctx, g);
     if (!twx || !twy || !twz || !eig) return fail(ERR_ALLOC, "table allocation failed");
+    if (U && prec != 32) return fail(ERR_INVALID, "residual pass is FP32 only");
 
     const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(T)), P1 = NY + 2;
     const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
@@ -150,12 +159,14 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
     sa.pertrb = pertrb;
     sa.pert_scale = pertrb ? mode0_inv_wsum(sa) : 0.0;
     int rc;
-    hipError_t e;
-    const bool tm = ctx.timing();
 
     // pass 1: folds + forward x, f rows (j,k) of the unknown box -> ex
     {
-        PassArgs p{};
+        PassSpec q{};
+        q.op = U ? B_XRES : B_XFWD;
+        q.prec = prec; q.ept = cx.EPT; q.N = NX; q.role = ROLE_XFWD;
+        q.name = U ? "xres" : "xfwd";
+        PassArgs &p = q.d.p;
         fill_pass(p, cx, prec, g.fwd[0], twx);
         p.src = f; p.dst = ex;
         p.s_sa = fx; p.s_sb = fx * fy; p.s_sp = 1; p.s_rowlen = (int)fx;
@@ -164,23 +175,17 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
         p.in_lo = g.s[0]; p.in_hi = g.e[0];
         p.st_lo = sxl; p.st_hi = sxh;
         p.out_lo = sxl; p.out_hi = sxh;
-        const int nb = g.e[2] - g.s[2] + 1;
-        if (!pass_in_bounds(p, true, nb, fvol, vol)) return fail(ERR_INVALID, "pass 1 geometry outside the buffers");
-        if (U) {
-            if (prec != 32) return fail(ERR_INVALID, "residual pass is FP32 only");
-            if (tm) ctx.timing_mark("xres", st);
-            if ((e = launch_pass_sa_ptr(solve_xres_kernel_f32(cx.EPT), p, sa, U, prec, nb, st)) != hipSuccess)
-                return fail(ERR_DEVICE, std::string("residual pass launch: ") + hipGetErrorString(e));
-        } else {
-            const void *fn = prec == 64 ? solve_xfwd_kernel_f64(cx.EPT) : solve_xfwd_kernel_f32(cx.EPT);
-            if (tm) ctx.timing_mark("xfwd", st);
-            if ((e = launch_solve_pass(fn, p, sa, prec, nb, st)) != hipSuccess)
-                return fail(ERR_DEVICE, std::string("pass 1 launch: ") + hipGetErrorString(e));
-        }
+        q.nb = g.e[2] - g.s[2] + 1;
+        q.d.sa = sa;
+        q.d.u_in = U;
+        if ((rc = check_spec(q, fvol, vol, "pass 1"))) return rc;
+        out.push_back(q);
     }
     // pass 2: forward y, columns x in the x spectral window, z in the unknown window
     {
-        PassArgs p{};
+        PassSpec q{};
+        q.op = B_COLS; q.prec = prec; q.ept = cy.EPT; q.N = NY; q.role = ROLE_COLS; q.name = "yfwd";
+        PassArgs &p = q.d.p;
         fill_pass(p, cy, prec, g.fwd[1], twy);
         p.src = ex; p.dst = ex;
         p.s_sa = 1; p.s_sb = P0 * P1; p.s_sp = P0;
@@ -189,12 +194,15 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
         p.in_lo = g.s[1]; p.in_hi = g.e[1];
         p.st_lo = syl; p.st_hi = syh;
         p.out_lo = syl; p.out_hi = syh;
-        if (tm) ctx.timing_mark("yfwd", st);
-        if ((rc = launch_checked(p, prec, cy.EPT, false, g.e[2] - g.s[2] + 1, vol, vol, st, "pass 2"))) return rc;
+        q.nb = g.e[2] - g.s[2] + 1;
+        if ((rc = check_spec(q, vol, vol, "pass 2"))) return rc;
+        out.push_back(q);
     }
     // pass 3: forward z, divide, inverse z
     {
-        PassArgs p{};
+        PassSpec q{};
+        q.op = B_ZMID; q.prec = prec; q.ept = cz.EPT; q.N = NZ; q.role = ROLE_ZMID; q.name = "zmid";
+        PassArgs &p = q.d.p;
         fill_pass(p, cz, prec, g.fwd[2], twz);
         p.src = ex; p.dst = ex;
         p.s_sa = 1; p.s_sb = P0; p.s_sp = P0 * P1;
@@ -203,20 +211,19 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
         p.in_lo = g.s[2]; p.in_hi = g.e[2];
         p.st_lo = g.s[2]; p.st_hi = g.e[2];
         p.out_lo = szl; p.out_hi = szh;
-        SolveArgs s3 = sa;
-        s3.kind2 = g.inv[2];
-        s3.out2_lo = g.s[2];
-        s3.out2_hi = g.e[2];
-        const int nb = syh - syl + 1;
-        if (!pass_in_bounds(p, false, nb, vol, vol)) return fail(ERR_INVALID, "pass 3 geometry outside the buffers");
-        const void *fn = prec == 64 ? solve_zmid_kernel_f64(cz.EPT) : solve_zmid_kernel_f32(cz.EPT);
-        if (tm) ctx.timing_mark("zmid", st);
-        if ((e = launch_solve_pass(fn, p, s3, prec, nb, st)) != hipSuccess)
-            return fail(ERR_DEVICE, std::string("pass 3 launch: ") + hipGetErrorString(e));
+        q.d.sa = sa;
+        q.d.sa.kind2 = g.inv[2];
+        q.d.sa.out2_lo = g.s[2];
+        q.d.sa.out2_hi = g.e[2];
+        q.nb = syh - syl + 1;
+        if ((rc = check_spec(q, vol, vol, "pass 3"))) return rc;
+        out.push_back(q);
     }
     // pass 4: inverse y
     {
-        PassArgs p{};
+        PassSpec q{};
+        q.op = B_COLS; q.prec = prec; q.ept = cy.EPT; q.N = NY; q.role = ROLE_COLS; q.name = "yinv";
+        PassArgs &p = q.d.p;
         fill_pass(p, cy, prec, g.inv[1], twy);
         p.src = ex; p.dst = ex;
         p.s_sa = 1; p.s_sb = P0 * P1; p.s_sp = P0;
@@ -225,28 +232,36 @@ int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t s
         p.in_lo = syl; p.in_hi = syh;
         p.st_lo = g.s[1]; p.st_hi = g.e[1];
         p.out_lo = g.s[1]; p.out_hi = g.e[1];
-        if (tm) ctx.timing_mark("yinv", st);
-        if ((rc = launch_checked(p, prec, cy.EPT, false, g.e[2] - g.s[2] + 1, vol, vol, st, "pass 4"))) return rc;
+        q.nb = g.e[2] - g.s[2] + 1;
+        if ((rc = check_spec(q, vol, vol, "pass 4"))) return rc;
+        out.push_back(q);
     }
     return OK;
 }
 
-// pass 5 of the plain solve: inverse x of the ex rows, x ratio, store into f
+// Last pass: inverse x of the workspace rows and the x ratio.  FP64: stored
+// into f over the unknown box (out == f, Uin == nullptr).  FP32: out = Uin +
+// ratio * delta in FP64 (Uin == nullptr: out = ratio * delta).
 template <typename T>
-int solve_store(Context &ctx, const SolveGeom &g, double *f, const T *ex, hipStream_t st)
+int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx;
     if (!ctx.config(NX, prec, ROLE_ROWS, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
     const void *twx = ctx.twiddles(NX, prec);
+    if (!twx) return fail(ERR_ALLOC, "table allocation failed");
     const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(T)), P1 = NY + 2;
     const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
     int sxl, sxh;
     out_window(g.fwd[0], NX, &sxl, &sxh);
-    PassArgs p{};
+    PassSpec q{};
+    q.op = prec == 64 ? B_ROWS : B_XUPD;
+    q.prec = prec; q.ept = cx.EPT; q.N = NX; q.role = ROLE_ROWS;
+    q.name = Uin ? "xupd" : "xinv";
+    PassArgs &p = q.d.p;
     fill_pass(p, cx, prec, g.inv[0], twx);
-    p.src = ex; p.dst = f;
+    p.src = ex; p.dst = out;
     p.s_sa = P0; p.s_sb = P0 * P1; p.s_sp = 1; p.s_rowlen = (int)P0;
     p.d_sa = fx; p.d_sb = fx * fy; p.d_sp = 1; p.d_rowlen = (int)fx;
     p.a0 = g.s[1]; p.na = g.e[1] - g.s[1] + 1; p.b0 = g.s[2];
@@ -254,10 +269,57 @@ int solve_store(Context &ctx, const SolveGeom &g, double *f, const T *ex, hipStr
     p.st_lo = g.s[0]; p.st_hi = g.e[0];
     p.out_lo = g.s[0]; p.out_hi = g.e[0];
     p.scale = g.ratio;
-    static_assert(sizeof(T) == 8, "the FP64 store pass writes the solution directly");
-    if (ctx.timing()) ctx.timing_mark("xinv", st);
-    return launch_checked(p, prec, cx.EPT, true, g.e[2] - g.s[2] + 1, vol, fvol, st, "pass 5");
+    q.nb = g.e[2] - g.s[2] + 1;
+    q.d.u_in = Uin;
+    q.d.u_out = out;
+    if (prec == 64 && Uin) return fail(ERR_INVALID, "the FP64 store pass writes the solution directly");
+    int rc;
+    if ((rc = check_spec(q, vol, fvol, "last pass"))) return rc;
+    v.push_back(q);
+    return OK;
 }
+
+// Launch one pass of a single solve with the single-problem kernels.
+int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st)
+{
+    if (ctx.timing()) ctx.timing_mark(q.name, st);
+    const PassArgs &p = q.d.p;
+    const bool f64 = q.prec == 64;
+    hipError_t e = hipErrorInvalidValue;
+    switch (q.op) {
+        case B_ROWS:
+        case B_COLS: e = launch_axis_pass(p, q.prec, q.ept, q.op == B_ROWS, q.nb, st); break;
+        case B_XFWD:
+            e = launch_solve_pass(f64 ? solve_xfwd_kernel_f64(q.ept) : solve_xfwd_kernel_f32(q.ept), p, q.d.sa, q.prec, q.nb, st);
+            break;
+        case B_ZMID:
+            e = launch_solve_pass(f64 ? solve_zmid_kernel_f64(q.ept) : solve_zmid_kernel_f32(q.ept), p, q.d.sa, q.prec, q.nb, st);
+            break;
+        case B_XRES:
+            if (!f64) e = launch_pass_sa_ptr(solve_xres_kernel_f32(q.ept), p, q.d.sa, q.d.u_in, q.prec, q.nb, st);
+            break;
+        case B_XUPD:
+            if (!f64) e = launch_pass_ptrs(solve_xinv_upd_kernel_f32(q.ept), p, q.d.u_in, q.d.u_out, q.prec, q.nb, st);
+            break;
+    }
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string(q.name) + " pass launch: " + hipGetErrorString(e));
+    return OK;
+}
+
+static int run_specs(Context &ctx, const std::vector<PassSpec> &v, hipStream_t st)
+{
+    int rc;
+    for (const PassSpec &q : v)
+        if ((rc = launch_spec(ctx, q, st))) return rc;
+    return OK;
+}
+
+template int solve_passes<double>(Context &, const SolveGeom &, double *, double *, const double *, const double *,
+                                  std::vector<PassSpec> &);
+template int solve_passes<float>(Context &, const SolveGeom &, double *, float *, const double *, const double *,
+                                 std::vector<PassSpec> &);
+template int final_pass<double>(Context &, const SolveGeom &, const double *, const double *, double *, std::vector<PassSpec> &);
+template int final_pass<float>(Context &, const SolveGeom &, const float *, const double *, double *, std::vector<PassSpec> &);
 
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st)
 {
@@ -286,42 +348,13 @@ int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const d
     const size_t bytes = sizeof(double) * (size_t)ws_row(N[0], sizeof(double)) * (N[1] + 2) * (N[2] + 2);
     double *ex = reinterpret_cast<double *>(ctx.workspace(bytes, 0));
     if (!ex) return fail(ERR_ALLOC, "workspace allocation failed");
+    std::vector<PassSpec> v;
+    if ((rc = solve_passes<double>(ctx, g, f, ex, nullptr, nullptr, v)) || (rc = final_pass<double>(ctx, g, ex, nullptr, f, v)))
+        return rc;
     PassTimer timer(ctx, st);
-    if ((rc = solve_core<double>(ctx, g, f, ex, st, nullptr, nullptr))) return rc;
-    if ((rc = solve_store<double>(ctx, g, f, ex, st))) return rc;
+    if ((rc = run_specs(ctx, v, st))) return rc;
     timer.mark("periodic");
     return periodic_copies(g, f, st);
-}
-
-// Last pass of the mixed-precision solve: inverse x of the FP32 workspace
-// rows, out = Uin + ratio * delta (FP64) over the unknown box.
-static int solve_update(Context &ctx, const SolveGeom &g, const float *ex, const double *Uin, double *out,
-                        hipStream_t st)
-{
-    const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
-    AxisCfg cx;
-    if (!ctx.config(NX, 32, ROLE_ROWS, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
-    const void *twx = ctx.twiddles(NX, 32);
-    const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(float)), P1 = NY + 2;
-    const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
-    int sxl, sxh;
-    out_window(g.fwd[0], NX, &sxl, &sxh);
-    PassArgs p{};
-    fill_pass(p, cx, 32, g.inv[0], twx);
-    p.src = ex; p.dst = out;
-    p.s_sa = P0; p.s_sb = P0 * P1; p.s_sp = 1; p.s_rowlen = (int)P0;
-    p.d_sa = fx; p.d_sb = fx * fy; p.d_sp = 1; p.d_rowlen = (int)fx;
-    p.a0 = g.s[1]; p.na = g.e[1] - g.s[1] + 1; p.b0 = g.s[2];
-    p.in_lo = sxl; p.in_hi = sxh;
-    p.st_lo = g.s[0]; p.st_hi = g.e[0];
-    p.out_lo = g.s[0]; p.out_hi = g.e[0];
-    p.scale = g.ratio;
-    const int nb = g.e[2] - g.s[2] + 1;
-    if (!pass_in_bounds(p, true, nb, vol, fvol)) return fail(ERR_INVALID, "update pass geometry outside the buffers");
-    if (ctx.timing()) ctx.timing_mark(Uin ? "xupd" : "xinv", st);
-    hipError_t e = launch_pass_ptrs(solve_xinv_upd_kernel_f32(cx.EPT), p, Uin, out, 32, nb, st);
-    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("update pass launch: ") + hipGetErrorString(e));
-    return OK;
 }
 
 // FP32 transforms and divide with FP64 iterative refinement: an FP32 solve
@@ -355,12 +388,15 @@ int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const 
         hipError_t e = launch_fold_sum(sa0, acc + 1, st);
         if (e != hipSuccess) return fail(ERR_DEVICE, std::string("PERTRB launch: ") + hipGetErrorString(e));
     }
-    if ((rc = solve_core<float>(ctx, g, f, ex, st, nullptr, nullptr))) return rc;
-    if ((rc = solve_update(ctx, g, ex, nullptr, refine > 0 ? U : f, st))) return rc;
-    for (int it = 1; it <= refine; it++) {
-        if ((rc = solve_core<float>(ctx, g, f, ex, st, U, acc ? acc + 1 : nullptr))) return rc;
-        if ((rc = solve_update(ctx, g, ex, U, it == refine ? f : U, st))) return rc;
-    }
+    std::vector<PassSpec> v;
+    if ((rc = solve_passes<float>(ctx, g, f, ex, nullptr, nullptr, v)) ||
+        (rc = final_pass<float>(ctx, g, ex, nullptr, refine > 0 ? U : f, v)))
+        return rc;
+    for (int it = 1; it <= refine; it++)
+        if ((rc = solve_passes<float>(ctx, g, f, ex, U, acc ? acc + 1 : nullptr, v)) ||
+            (rc = final_pass<float>(ctx, g, ex, U, it == refine ? f : U, v)))
+            return rc;
+    if ((rc = run_specs(ctx, v, st))) return rc;
     if (acc) {
         // the corrections never change the 0-mode of the FP32 solve (rounding
         // size): remove it so the result has the zero mode of the FP64 path

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "chz_context.h"
 
 namespace chz {
@@ -22,9 +24,24 @@ int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3]
 const double *solver_eig(Context &ctx, const SolveGeom &g);
 void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const double *eig);
 
-// passes 1-4 of a solve into workspace ex (rhs read from f)
+// One pass of a solve: the kernel (BatchOp), its launch configuration and
+// its arguments; launched alone (launch_spec) or grouped with the same pass
+// of other problems of the same size class (batched solves).
+struct PassSpec {
+    int op, prec, ept, nb;   // nb: line groups (tile index b)
+    int N, role;             // cell count along the pass axis, configuration role
+    const char *name;
+    BatchDesc d;
+};
+
+// passes 1-4 of a solve into workspace ex (rhs read from f; U: residual of U instead)
 template <typename T>
-int solve_core(Context &ctx, const SolveGeom &g, double *f, T *ex, hipStream_t st, const double *U, const double *pertrb);
+int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
+                 std::vector<PassSpec> &out);
+// inverse x pass into out (FP64: the solution; FP32: out = Uin + delta)
+template <typename T>
+int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v);
+int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st);
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 
 int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,

@@ -90,3 +90,16 @@ def batched_problems(count=2048, seed=2, sizes=(32, 48, 64, 96, 128), delta=1e-3
             break
         out.append({"n": n, "bc": bc, "lam": lam, "seed": int(rng.integers(0, 2 ** 31))})
     return out
+
+
+def problem_arrays(desc):
+    """f ~ N(0,1) on the nodes and N(0,1) Neumann face data (where the codes
+    need it) of one batched problem, from its own seed."""
+    NX, NY, NZ = desc["n"]
+    bc = desc["bc"]
+    rng = np.random.default_rng(desc["seed"])
+    f = rng.standard_normal((NZ + 1, NY + 1, NX + 1))
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    bcs = [rng.standard_normal(sizes[i]) if bc[i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None
+           for i in range(6)]
+    return f, bcs

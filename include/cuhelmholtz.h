@@ -70,6 +70,40 @@ int chz_helmholtz_solve_mixed(double left, double right, int NX, int xbc, const 
                               double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
                               double lambda, double *f, int refine, void *stream);
 
+/* One problem of a batched solve: the arguments of chz_helmholtz_solve. */
+typedef struct chz_problem {
+    double domain[6];        /* left, right, bottom, top, front, back */
+    int N[3];                /* NX, NY, NZ cells */
+    int bc[3];               /* boundary codes x, y, z (0..4) */
+    double lambda;
+    double *f;               /* (NX+1)(NY+1)(NZ+1) node array: rhs in, solution out */
+    const double *faces[6];  /* bcl, bcr, bcb, bct, bcf, bce (NULL where unused) */
+} chz_problem;
+
+/* Plan a batched solve of `count` independent problems (any mix of sizes and
+ * boundary codes): precision 64 = plain FP64 (refine must be 0), precision
+ * 32 = FP32 transforms + `refine` FP64 refinement steps as in
+ * chz_helmholtz_solve_mixed (singular problems then need refine == 0).  The
+ * plan owns one workspace per problem and device tables of per-problem pass
+ * descriptors, and holds the whole solve captured as one hipGraph: one
+ * persistent launch per pass and size class, then the periodic copies.  The
+ * problems' f arrays and face arrays are bound here: each chz_batch_solve
+ * solves whatever they hold at that time.  Results equal the single-problem
+ * solve's (same kernels and launch configurations).  No reference
+ * counterpart (the reference solves one problem per call). */
+int chz_batch_create(const chz_problem *problems, int count, int precision, int refine, void **plan);
+
+/* Run the batched solve on `stream`: use_graph = 1 replays the captured
+ * graph (one hipGraphLaunch); 0 issues the same launches directly (and
+ * records per-launch timing when chz_set_timing is on). */
+int chz_batch_solve(void *plan, int use_graph, void *stream);
+
+/* Launch count of one batched solve and the plan's device memory. */
+int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *table_bytes);
+
+/* Free the plan (synchronise any stream running it first). */
+int chz_batch_destroy(void *plan);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *chz_last_error(void);
 

@@ -1,0 +1,157 @@
+"""GPU parity of the batched solve (one hipGraph, one persistent launch per
+pass and size class, descriptor tables), through the C ABI.
+
+Bar: every problem of a batch equals the single-problem solve of the same
+problem (same kernels bodies and launch configurations) to 1e-13 relative
+L-infinity, FP64 and FP32+refine; FP64 problems match the oracle's
+restatement of cuda_helmholtz_solver on a sample to 1e-12; graph replays,
+direct launches and repeated replays agree.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not torch.cuda.is_available(), reason="no GPU")]
+
+import cuhelmholtz_amd as C  # noqa: E402
+from cuhelmholtz_amd import workloads as W  # noqa: E402
+
+SAME = 1e-13
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def rel(a, b):
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+
+def make(descs):
+    """device rhs tensors, face tensors and host copies of a problem list"""
+    out = []
+    for d in descs:
+        f, bcs = W.problem_arrays(d)
+        ft = torch.from_numpy(f).to(dev())
+        bt = [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs]
+        out.append({"d": d, "f0": ft.clone(), "f": ft, "bcs": bt, "f_np": f, "bcs_np": bcs})
+    return out
+
+
+def small_descs(count, seed, sizes=(8, 12, 16, 20, 24)):
+    return W.batched_problems(count=count, seed=seed, sizes=sizes)
+
+
+def single(p, prec=64, refine=0):
+    u = p["f0"].clone()
+    d = p["d"]
+    if prec == 64:
+        C.helmholtz_solve(u, d["n"], d["bc"], d["lam"], p["bcs"])
+    else:
+        C.helmholtz_solve_mixed(u, d["n"], d["bc"], d["lam"], p["bcs"], refine=refine)
+    return u
+
+
+def run_batch(probs, prec=64, refine=0, graph=True):
+    bs = C.BatchSolver([(p["f"], p["d"]["n"], p["d"]["bc"], p["d"]["lam"], p["bcs"]) for p in probs], prec, refine)
+    bs.solve(graph=graph)
+    torch.cuda.synchronize()
+    return bs
+
+
+def test_batch_fp64_matches_single_small():
+    probs = make(small_descs(40, 11))
+    bs = run_batch(probs)
+    info = bs.info()
+    assert info["launches"] >= 5
+    for k, p in enumerate(probs):
+        e = rel(p["f"], single(p))
+        assert e <= SAME, (k, p["d"], e)
+    bs.close()
+
+
+def test_batch_fp64_vs_oracle_sample():
+    probs = make(small_descs(20, 12))
+    bs = run_batch(probs)
+    for p in probs[::3]:
+        d = p["d"]
+        want = O.helmholtz(p["f_np"], d["n"], d["bc"], d["lam"], p["bcs_np"])
+        got = p["f"].cpu().numpy()
+        e = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+        assert e <= 1e-12, (d, e)
+    bs.close()
+
+
+def test_batch_graph_replays_and_eager_agree():
+    probs = make(small_descs(24, 13))
+    bs = C.BatchSolver([(p["f"], p["d"]["n"], p["d"]["bc"], p["d"]["lam"], p["bcs"]) for p in probs])
+    want = [single(p) for p in probs]
+    face0 = [[None if b is None else b.clone() for b in p["bcs"]] for p in probs]
+    for rep in range(4):
+        # new data in the bound arrays (rhs and Neumann faces scaled together:
+        # the solve is linear in both), then solve: graph on even reps, direct
+        # launches on odd ones
+        for k, p in enumerate(probs):
+            p["f"].copy_(p["f0"] * (1.0 + rep))
+            for b, b0 in zip(p["bcs"], face0[k]):
+                if b is not None:
+                    b.copy_(b0 * (1.0 + rep))
+        torch.cuda.synchronize()
+        bs.solve(graph=rep % 2 == 0)
+        torch.cuda.synchronize()
+        for k, p in enumerate(probs):
+            e = rel(p["f"], want[k] * (1.0 + rep))
+            assert e <= SAME, (rep, k, e)
+    bs.close()
+
+
+def test_batch_benchmark_sizes_sample():
+    """64 problems drawn like the 2048-problem workload (sizes 32..128)."""
+    probs = make(W.batched_problems(count=64, seed=2))
+    bs = run_batch(probs)
+    for k, p in enumerate(probs):
+        e = rel(p["f"], single(p))
+        assert e <= SAME, (k, p["d"], e)
+    bs.close()
+
+
+@pytest.mark.parametrize("refine", [0, 2])
+def test_batch_mixed_matches_single(refine):
+    probs = make(small_descs(24, 14, sizes=(16, 24, 32, 48)))
+    bs = run_batch(probs, prec=32, refine=refine)
+    for k, p in enumerate(probs):
+        e = rel(p["f"], single(p, 32, refine))
+        assert e <= SAME, (k, p["d"], e)
+        if refine == 2:
+            assert rel(p["f"], single(p)) <= 1e-9, (k, p["d"])
+    bs.close()
+
+
+def test_batch_singular_fp64_and_periodic():
+    descs = [{"n": (8, 12, 16), "bc": (0, 3, 0), "lam": 0.0, "seed": 5},
+             {"n": (16, 8, 8), "bc": (0, 0, 0), "lam": -2.0, "seed": 6},
+             {"n": (12, 12, 12), "bc": (3, 3, 3), "lam": 0.0, "seed": 7},
+             {"n": (8, 16, 12), "bc": (1, 0, 2), "lam": 0.25, "seed": 8}]
+    probs = make(descs)
+    bs = run_batch(probs)
+    for p in probs:
+        d = p["d"]
+        want = O.helmholtz(p["f_np"], d["n"], d["bc"], d["lam"], p["bcs_np"])
+        got = p["f"].cpu().numpy()
+        assert float(np.max(np.abs(got - want)) / np.max(np.abs(want))) <= 1e-12, d
+        assert rel(p["f"], single(p)) <= SAME
+    bs.close()
+
+
+def test_batch_errors():
+    probs = make([{"n": (8, 8, 8), "bc": (3, 3, 3), "lam": 0.0, "seed": 1}])
+    p = probs[0]
+    with pytest.raises(C.CHZError, match="singular"):
+        C.BatchSolver([(p["f"], p["d"]["n"], p["d"]["bc"], 0.0, p["bcs"])], prec=32, refine=1)
+    with pytest.raises(C.CHZError, match="problem 1"):
+        C.BatchSolver([(p["f"], (8, 8, 8), (3, 3, 3), -1.0, p["bcs"]),
+                       (p["f"], (8, 8, 8), (7, 3, 3), -1.0, p["bcs"])])
+    with pytest.raises(C.CHZError):
+        C.BatchSolver([(p["f"], (8, 8, 8), (3, 3, 3), -1.0, p["bcs"])], prec=64, refine=1)
