@@ -62,6 +62,8 @@ def _declare_solver(L):
         L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve_mixed"):
         L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
+    if hasattr(L, "chz_helmholtz_adjoint"):
+        L.chz_helmholtz_adjoint.argtypes = ([d, d, i, i] * 3) + [d, vp, vp, vp] + [vp] * 6 + [vp, vp]
 
 
 class _Problem(ctypes.Structure):
@@ -167,6 +169,40 @@ def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
     _check(lib().chz_helmholtz_solve(*args, float(lam), ctypes.c_void_p(f.data_ptr()), _stream(f)))
     del keep
     return f
+
+
+def helmholtz_adjoint(u, gu, n, bc, lam, dom=(0.0, 1.0) * 3, faces=False, want_lambda=True):
+    """Vector-Jacobian product of helmholtz_solve (chz_helmholtz_adjoint).
+
+    u: forward solution (needed for the lambda gradient), gu = dL/du, both
+    CUDA float64 node tensors.  Returns (gf, face gradients (6-list, None
+    where the codes read no face data, or all None if faces=False),
+    glambda (0-d CUDA tensor or None))."""
+    import torch
+    NX, NY, NZ = (int(v) for v in n)
+    shape = (NZ + 1, NY + 1, NX + 1)
+    for t in (u, gu):
+        if t is not None and (not t.is_cuda or t.dtype != torch.float64 or tuple(t.shape) != shape
+                              or not t.is_contiguous()):
+            raise ValueError("u and gu must be contiguous CUDA float64 tensors shaped (NZ+1, NY+1, NX+1)")
+    gu = gu.contiguous()
+    gf = torch.empty_like(gu)
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    gfaces = [None] * 6
+    if faces:
+        for i in range(6):
+            a = i // 2
+            if bc[a] in ((3, 4) if i % 2 == 0 else (2, 3)):
+                gfaces[i] = torch.empty(sizes[i], dtype=torch.float64, device=gu.device)
+    glam = torch.empty((), dtype=torch.float64, device=gu.device) if want_lambda else None
+    args = []
+    for a, N in enumerate((NX, NY, NZ)):
+        args += [float(dom[2 * a]), float(dom[2 * a + 1]), N, int(bc[a])]
+    ptr = [None if t is None else ctypes.c_void_p(t.data_ptr()) for t in gfaces]
+    _check(lib().chz_helmholtz_adjoint(*args, float(lam), None if u is None else ctypes.c_void_p(u.data_ptr()),
+                                       ctypes.c_void_p(gu.data_ptr()), ctypes.c_void_p(gf.data_ptr()), *ptr,
+                                       None if glam is None else ctypes.c_void_p(glam.data_ptr()), _stream(gu)))
+    return gf, gfaces, glam
 
 
 def set_timing(on):

@@ -11,6 +11,7 @@ library loads at start-up from tuning/<gfx arch>.txt beside the shared object.
 Usage (on the GPU):
   python -m cuhelmholtz_amd.autotune --solve 512 512 512 --prec 64 [--out FILE]
   python -m cuhelmholtz_amd.autotune --transform 256 --prec 64
+  python -m cuhelmholtz_amd.autotune --batch 32 48 64 96 128 --prec 32
 """
 import argparse
 import os
@@ -151,6 +152,80 @@ def tune_transform(N, prec=64, log=print):
     return table
 
 
+def tune_batch(N, prec=64, count=32, reps=3, log=print):
+    """Tune the roles of one size class of the batched solve: a batch of
+    `count` N^3 problems (codes and lambda drawn like the batched workload),
+    timed per launch through direct (non-graph) batched solves."""
+    import numpy as np
+    import torch
+    from cuhelmholtz_amd import workloads as W
+    dev = torch.device("cuda:0")
+    descs = W.batched_problems(count=count, seed=7, sizes=(N,))
+    probs, f0 = [], []
+    for d in descs:
+        f, bcs = W.problem_arrays(d)
+        ft = torch.from_numpy(f).to(dev)
+        f0.append(ft.clone())
+        probs.append((ft, d["n"], d["bc"], d["lam"], [None if b is None else torch.from_numpy(b).to(dev) for b in bcs]))
+    refine = 0 if prec == 64 else 1
+    C.reset_axis_configs()
+
+    def solve_into():
+        for p, g in zip(probs, f0):
+            p[0].copy_(g)
+        bs = C.BatchSolver(probs, prec, refine)
+        bs.solve(graph=False)
+        torch.cuda.synchronize()
+        return bs
+
+    solve_into().close()
+    ref = [p[0].clone() for p in probs]
+    tol = 1e-12 if prec == 64 else 1e-9
+
+    def check():
+        solve_into().close()
+        return all(float((p[0] - r).abs().max() / r.abs().max()) < tol for p, r in zip(probs, ref))
+
+    def timed(names):
+        for p, g in zip(probs, f0):
+            p[0].copy_(g)
+        bs = C.BatchSolver(probs, prec, refine)
+        bs.solve(graph=False)
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(reps):
+            C.set_timing(True)
+            bs.solve(graph=False)
+            torch.cuda.synchronize()
+            rec = C.get_timing(4096)
+            C.set_timing(False)
+            out.append(sum(ms for nm, ms in rec if nm.split("/")[0] in names))
+        bs.close()
+        return float(np.median(out))
+
+    plan = [(C.ROLE_COLS, ("yfwd", "yinv")), (C.ROLE_ZMID, ("zmid",)), (C.ROLE_XFWD, ("xfwd", "xres")),
+            (C.ROLE_ROWS, ("xinv", "xupd"))]
+    table = []
+    for role, names in plan:
+        results = []
+        for ept, tpl, lines in candidates(N):
+            try:
+                C.set_axis_config(N, prec, role, ept, tpl, lines)
+            except C.CHZError:
+                continue
+            if not check():
+                log(f"  reject {ept} {tpl} {lines}: result differs")
+                continue
+            results.append((timed(names), ept, tpl, lines))
+        results.sort()
+        best = results[0]
+        C.set_axis_config(N, prec, role, best[1], best[2], best[3])
+        log(f"batch role {C.ROLE_NAMES[role]} N={N} prec={prec}: best ept={best[1]} tpl={best[2]} lines={best[3]} "
+            f"{best[0]:.3f} ms ({len(results)} candidates; worst {results[-1][0]:.3f} ms)")
+        table.append((role, N, prec, best[1], best[2], best[3], best[0], results))
+    return table
+
+
 def write_table(path, rows, header):
     """merge rows into a tuning table file (later rows win per (role, N, prec))"""
     entries = {}
@@ -173,6 +248,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--solve", type=int, nargs=3)
     ap.add_argument("--transform", type=int)
+    ap.add_argument("--batch", type=int, nargs="*", help="size classes of the batched solve to tune")
     ap.add_argument("--prec", type=int, default=64)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning", "gfx950.txt"))
     args = ap.parse_args()
@@ -182,6 +258,8 @@ def main():
         rows += tune_solve(tuple(args.solve), args.prec)
     if args.transform:
         rows += tune_transform(args.transform, args.prec)
+    for N in args.batch or []:
+        rows += tune_batch(N, args.prec)
     header = ("# launch configurations chosen by cuhelmholtz_amd.autotune on MI355X (gfx950)\n"
               "# role N prec ept tpl lines   (role: 0 rows, 1 cols, 2 solve xfwd, 3 solve zmid, 4 z cols)\n")
     write_table(args.out, rows, header)

@@ -83,6 +83,15 @@ struct BatchCopy {
 
 
 
+// Geometry of an adjoint solve (node array (NX+1)(NY+1)(NZ+1), x fastest).
+struct AdjArgs {
+    int N[3], bc[3], s[3], e[3];
+    double ih[3], ih2[3];
+    double *face[6];   // gradient face arrays (null where not wanted)
+};
+// adjoint kernels: 0 prep, 1 finish, 2 lambda reduction, 3 zero fill
+const void *adj_kernel(int which);
+
 // kernel getters (one translation unit per precision)
 const void *axis_kernel_f64_rows(int ept);
 const void *axis_kernel_f64_cols(int ept);

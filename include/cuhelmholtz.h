@@ -70,6 +70,24 @@ int chz_helmholtz_solve_mixed(double left, double right, int NX, int xbc, const 
                               double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
                               double lambda, double *f, int refine, void *stream);
 
+/* Adjoint (vector-Jacobian product) of chz_helmholtz_solve for the same
+ * geometry, codes and lambda.  The solve maps (f, Neumann face data) to the
+ * node array u linearly; given gu = dL/du on all (NX+1)(NY+1)(NZ+1) nodes it
+ * writes gf = dL/df (same layout; 0 at the periodic duplicate nodes, which
+ * the solve overwrites), the face gradients gbcl..gbce (layouts of the face
+ * arrays; NULL = not wanted, and not written where the codes read no face
+ * data) and, if glambda is set, glambda[0] = dL/dlambda = -<A^-T gu, u> on
+ * the unknown box, which needs u = the forward solution.  All pointers are
+ * device pointers; gf must not alias gu.  Runs one FP64 solve (A^-T =
+ * W A^-1 W^-1, W = 1/2 at Neumann end nodes) between two node-array
+ * passes.  No reference counterpart (the reference has no derivatives). */
+int chz_helmholtz_adjoint(double left, double right, int NX, int xbc,
+                          double bottom, double top, int NY, int ybc,
+                          double front, double back, int NZ, int zbc,
+                          double lambda, const double *u, const double *gu, double *gf,
+                          double *gbcl, double *gbcr, double *gbcb, double *gbct, double *gbcf, double *gbce,
+                          double *glambda, void *stream);
+
 /* One problem of a batched solve: the arguments of chz_helmholtz_solve. */
 typedef struct chz_problem {
     double domain[6];        /* left, right, bottom, top, front, back */
