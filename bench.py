@@ -21,7 +21,12 @@ Prints ONE JSON line (rank 0):
   cpu_baseline  the oracle's hw3crt-contract CPU solver (oracle/chz_oracle.c)
               on a 512x512x64 slab of the same workload, all host cores,
               scaled x1/8 to a 512^3 solve
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  config.batched  2048 independent solves (sizes 32..128 per axis, codes
+              0-4, lambda in [-10, 0.5]) replayed as one hipGraph, FP64 and
+              FP32 + refinement, against the same problems solved one by one
+  config.adjoint  forward and adjoint backward of the differentiable solve
+              (batch 16 of 128^3, codes (1,3,0), lambda = -2)
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-adjoint]
 """
 import argparse
 import json
@@ -95,6 +100,140 @@ def cpu_baseline(n, bc, lam):
                       f"median of {len(times)} runs ({t:.2f} s each), x{scale:g} to {n[0]}x{n[1]}x{n[2]}"}
 
 
+def batched_workload(count, dev):
+    """The batched workload on the device: per-problem sizes from
+    {32,48,64,96,128}, codes 0-4, lambda ~ U[-10,0.5] (singular and
+    near-resonant values excluded), f and Neumann face data ~ N(0,1) drawn on
+    the device from each problem's seed."""
+    import torch
+    from cuhelmholtz_amd import workloads as W
+    descs = W.batched_problems(count=count, seed=2)
+    probs, f0 = [], []
+    for d in descs:
+        NX, NY, NZ = d["n"]
+        g = torch.Generator(device=dev)
+        g.manual_seed(d["seed"])
+        f = torch.randn((NZ + 1, NY + 1, NX + 1), generator=g, dtype=torch.float64, device=dev)
+        sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+        bcs = [torch.randn(sizes[i], generator=g, dtype=torch.float64, device=dev)
+               if d["bc"][i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None for i in range(6)]
+        probs.append((f, d["n"], d["bc"], d["lam"], bcs))
+        f0.append(f.clone())
+    return descs, probs, f0
+
+
+def bench_batched(dev, steps, count=2048):
+    """2048 independent solves in one hipGraph: FP64, and FP32 + the fewest
+    refinement steps reaching 1e-10 against FP64 on every problem."""
+    import numpy as np
+    import torch
+    import cuhelmholtz_amd as C
+    descs, probs, f0 = batched_workload(count, dev)
+    cells = sum(d["n"][0] * d["n"][1] * d["n"][2] for d in descs)
+
+    def reset():
+        for p, g in zip(probs, f0):
+            p[0].copy_(g)
+
+    def timed(bs):
+        bs.solve()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            bs.solve()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    bs = C.BatchSolver(probs, 64, 0)
+    info = bs.info()
+    ms = timed(bs)
+    reset()
+    bs.solve()
+    torch.cuda.synchronize()
+    ref = [p[0].clone() for p in probs]
+    bs.close()
+    # the same problems as sequential single-problem calls
+    reset()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for p in probs:
+        C.helmholtz_solve(p[0], p[1], p[2], p[3], p[4])
+    e1.record()
+    torch.cuda.synchronize()
+    ms_seq = e0.elapsed_time(e1)
+    same = max(float((p[0] - r).abs().max() / r.abs().max()) for p, r in zip(probs, ref))
+    log(f"batched fp64: {count} problems, {cells / 1e6:.0f} M cells: {ms:.3f} ms ({count / ms * 1e3:.0f} problems/s, "
+        f"{80.0 * cells / ms / 1e6:.0f} GB/s model), {info['launches']} launches in one graph; sequential single "
+        f"solves {ms_seq:.1f} ms (max rel diff {same:.1e})")
+    out = {"workload": f"{count} problems, N per axis in {{32,48,64,96,128}}, codes 0-4, lambda in [-10,0.5], "
+                       f"f ~ N(0,1) (seed 2)", "cells": cells, "launches_per_graph": info["launches"],
+           "fp64": {"ms": ms, "problems_per_s": count / ms * 1e3, "hbm_gbs_model": 80.0 * cells / ms / 1e6,
+                    "sequential_ms": ms_seq, "speedup_vs_sequential": ms_seq / ms, "max_rel_diff_vs_single": same},
+           "fp32_refine": None}
+    for k in range(0, 4):
+        reset()
+        bm = C.BatchSolver(probs, 32, k)
+        bm.solve()
+        torch.cuda.synchronize()
+        errs = np.array([float((p[0] - r).abs().max() / r.abs().max()) for p, r in zip(probs, ref)])
+        if errs.max() <= 1e-10:
+            mms = timed(bm)
+            log(f"batched fp32 + {k} refinement(s): {mms:.3f} ms ({ms / mms:.2f}x fp64), max rel err {errs.max():.1e}")
+            out["fp32_refine"] = {"refine": k, "ms": mms, "problems_per_s": count / mms * 1e3, "vs_fp64": ms / mms,
+                                  "max_rel_err_vs_fp64": float(errs.max()),
+                                  "median_rel_err_vs_fp64": float(np.median(errs))}
+            bm.close()
+            break
+        log(f"batched fp32 + {k} refinement(s): max rel err {errs.max():.1e} ({int((errs > 1e-10).sum())} problems "
+            f"above 1e-10)")
+        bm.close()
+    del probs, f0, ref
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adjoint(dev, steps=3):
+    """Differentiable solve: batch 16 of 128^3, codes (1,3,0), lambda = -2,
+    L = 1/2 |u - u_target|^2; forward and adjoint backward timed."""
+    import torch
+    from cuhelmholtz_amd.autograd import helmholtz
+    B, N = 16, 128
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    f = torch.randn((B, N + 1, N + 1, N + 1), generator=g, dtype=torch.float64, device=dev).requires_grad_(True)
+    g.manual_seed(4)
+    ut = torch.randn((B, N + 1, N + 1, N + 1), generator=g, dtype=torch.float64, device=dev)
+    fs = (N + 1) * (N + 1)
+    bcs = (None, None, torch.zeros(fs, dtype=torch.float64, device=dev), torch.zeros(fs, dtype=torch.float64, device=dev),
+           None, None)
+    lam = torch.tensor(-2.0, dtype=torch.float64, device=dev, requires_grad=True)
+    fw, bw = [], []
+    for it in range(steps + 1):
+        f.grad = None
+        lam.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u = helmholtz(f, lam, (1, 3, 0), bcs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L = 0.5 * ((u - ut) ** 2).sum()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if it:
+            fw.append(t1 - t0)
+            bw.append(t3 - t2)
+    fms, bms = 1e3 * min(fw), 1e3 * min(bw)
+    log(f"adjoint workload 16x128^3: forward {fms:.2f} ms, backward {bms:.2f} ms (wall, incl. host loop)")
+    return {"workload": "batch 16 of 128^3, codes (1,3,0), lambda -2, L = 1/2 |u - u_target|^2",
+            "forward_ms": fms, "backward_ms": bms, "backward_per_forward": bms / fms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +241,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, nargs=3, default=[512, 512, 512])
+    ap.add_argument("--no-batch", action="store_true", help="skip the 2048-problem batched solve")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-solve timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (single-GPU build)", flush=True)
@@ -210,6 +351,9 @@ def main():
     log(f"fp64 do_transform {kinds}: {tms:.3f} ms -> {1000 / tms:.0f} transforms/s, {48.0 * V / tms / 1e6:.0f} GB/s model")
     del x, y
 
+    batched = None if args.no_batch else bench_batched(dev, max(1, min(args.steps, 10)))
+    adjoint = None if args.no_adjoint else bench_adjoint(dev)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, bc, lam)
@@ -236,7 +380,9 @@ def main():
                    "fp64": {"solves_per_s": solves, "ms": ms, "relative_residual": rres,
                             "hbm_gbs_model": 80.0 * V / ms / 1e6, "pass_ms": avg},
                    "fp32_refine": mixed_info,
-                   "transforms_per_s_fp64": 1000.0 / tms},
+                   "transforms_per_s_fp64": 1000.0 / tms,
+                   "batched": batched,
+                   "adjoint": adjoint},
         "roofline": {"bound": "hbm", "kernel": PASS_KERNEL.get(dom, dom), "pass": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PASS_KERNEL.get(dom, dom)) if n == (512, 512, 512) else None},

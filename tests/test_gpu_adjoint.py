@@ -159,7 +159,7 @@ def test_autograd_central_differences(B, N):
         dl /= nrm
         fd = (loss(f + eps * df, -2.0 + eps * dl) - loss(f - eps * df, -2.0 - eps * dl)) / (2 * eps)
         an = float((gf * df).sum()) + gl * dl
-        worst = max(worst, abs(fd - an) / max(abs(an), 1e-3 * float(L)))
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-3 * float(L.detach())))
     assert worst <= 1e-5, worst
 
 
