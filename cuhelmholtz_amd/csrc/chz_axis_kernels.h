@@ -7,7 +7,7 @@
 namespace chz {
 
 // One fused axis pass over a tile of L lines: load -> fold -> FFT -> unfold -> store.
-template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
+template <typename T, typename SRC, typename DST, int EPT, bool ROWS, int CH = 0>
 __device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long long b, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
@@ -17,11 +17,11 @@ __device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long lo
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
-    load_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    load_tile<T, ROWS, SRC, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, ROWS>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
+    store_tile<T, ROWS, DST, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
 }
 
 template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
@@ -39,10 +39,10 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
 // Neumann    f[first] += 2 g_l / h,  f[last] -= 2 g_r / h
 // in the reference's order x, y, z (cuda_helmholtz_solver.cu:234-285).
 template <typename T>
-__device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<T> &tl, int nlines, int j0, int k)
+__device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<T> &tl, int nlines, int j0, int k, int TPL)
 {
-    const int t = threadIdx.x, nt = blockDim.x;
     const int L = tl.LS - 1;
+    const int t = threadIdx.x, nt = L * TPL;
     const int c = t % L, u0 = t / L, step = nt / L;
     if (c >= nlines) return;
     const int j = j0 + c;
@@ -88,7 +88,7 @@ __device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<
 
 // Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
 // folds, forward x transform, store into the padded spectral workspace.
-template <typename T, int EPT>
+template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa, int a_tile, int k, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
@@ -98,13 +98,13 @@ __device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
-    load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    load_tile<T, true, double, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
     __syncthreads();
-    fold_boundaries<T>(sa, tl, nlines, a_tile, k);
+    fold_boundaries<T>(sa, tl, nlines, a_tile, k, p.TPL);
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
 template <typename T, int EPT>
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, Solv
 // multiply by 1/(eig_x + eig_y + eig_z + lambda) (the reference's
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
 // inverse z transform, all in LDS.
-template <typename T, int EPT>
+template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa, int a_tile, int y, unsigned char *smem)
 {
     typedef typename vec2_of<T>::type V;
@@ -131,8 +131,8 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     const int u = (int)threadIdx.x / L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
-    load_tile<T, false>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    load_tile<T, false, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     {
@@ -154,7 +154,7 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     PassArgs q = p;
     q.out_lo = sa.out2_lo;
     q.out_hi = sa.out2_hi;
-    store_tile<T, false>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, false, T, CH>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
 template <typename T, int EPT>
@@ -227,7 +227,7 @@ __device__ __forceinline__ double residual_at(const SolveArgs &sa, const double 
 
 // Refinement pass 1: FP64 residual of the rows (j,k) of the unknown box for
 // the current solution U, rounded to T, forward x transform -> workspace.
-template <typename T, int EPT>
+template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa, const double *U, int a_tile, int k,
                                           unsigned char *smem)
 {
@@ -238,23 +238,23 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
     {
-        constexpr int CH = 8;
-        const int t = threadIdx.x, nt = blockDim.x, NS = p.N + 2;
+        constexpr int RC = 8;   // residuals in flight per thread
+        const int t = threadIdx.x, nt = p.L * p.TPL, NS = p.N + 2;
         T *rbuf = reinterpret_cast<T *>(tl.buf);
         const int total = L * NS;
-        for (int i0 = t; i0 < total; i0 += CH * nt) {
-            double v[CH];
+        for (int i0 = t; i0 < total; i0 += RC * nt) {
+            double v[RC];
 #pragma unroll
-            for (int i = 0; i < CH; i++) {
+            for (int i = 0; i < RC; i++) {
                 const int idx = i0 + i * nt;
                 const int r = idx / NS, sl = idx - (idx / NS) * NS;
                 v[i] = 0.0;
                 if (idx < total && r < nlines && sl >= sa.s[0] && sl <= sa.e[0]) v[i] = residual_at(sa, U, sl, a_tile + r, k);
             }
 #pragma unroll
-            for (int i = 0; i < CH; i++) {
+            for (int i = 0; i < RC; i++) {
                 const int idx = i0 + i * nt;
                 const int r = idx / NS, sl = idx - (idx / NS) * NS;
                 if (idx < total) rbuf[(size_t)((sl >> 1) * tl.LS + r) * 2 + (sl & 1)] = (T)v[i];
@@ -263,7 +263,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
     }
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, true>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
 }
 
 template <typename T, int EPT>
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, Solv
 // Last pass of a mixed-precision solve: inverse x transform of the workspace
 // rows, then out = Uin + ratio * delta in FP64 over the unknown x window
 // (Uin == nullptr: out = ratio * delta).
-template <typename T, int EPT>
+template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, double *out, int a_tile, int k,
                                           unsigned char *smem)
 {
@@ -289,23 +289,23 @@ __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, 
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M);
-    load_tile<T, true>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    load_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src));
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     {
-        constexpr int CH = 16;
-        const int t = threadIdx.x, nt = blockDim.x;
+        constexpr int UC = 16;   // updates in flight per thread
+        const int t = threadIdx.x, nt = p.L * p.TPL;
         const int RL = p.d_rowlen;
         const int total = nlines * RL;
         const long long base = (long long)a_tile * p.d_sa + (long long)k * p.d_sb;
         const T *rbuf = reinterpret_cast<const T *>(tl.buf);
         const double sc = p.scale;
-        for (int i0 = t; i0 < total; i0 += CH * nt) {
-            double v[CH];
-            bool w[CH];
+        for (int i0 = t; i0 < total; i0 += UC * nt) {
+            double v[UC];
+            bool w[UC];
 #pragma unroll
-            for (int i = 0; i < CH; i++) {
+            for (int i = 0; i < UC; i++) {
                 const int idx = i0 + i * nt;
                 const int r = idx / RL, sl = idx - (idx / RL) * RL;
                 w[i] = idx < total && sl >= p.st_lo && sl <= p.st_hi;
@@ -316,7 +316,7 @@ __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, 
                 }
             }
 #pragma unroll
-            for (int i = 0; i < CH; i++)
+            for (int i = 0; i < UC; i++)
                 if (w[i]) out[base + i0 + i * nt] = v[i];
         }
     }
@@ -405,6 +405,38 @@ __global__ void __launch_bounds__(256) k_batch_periodic(const C *__restrict__ ta
         else if (c.axis == 1) f[a + c.NY * fx + b * fx * fy] = f[a + b * fx * fy];
         else f[a + b * fx + c.NZ * fx * fy] = f[a + b * fx];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-shape passes: the same bodies with the cell count, threads per line,
+// lines per tile and radix plan as compile-time constants, so every index,
+// stride and twiddle offset folds into immediates and the FFT stages unroll
+// with constant spans (no integer division).  One kernel per (op, shape);
+// the host uses it when a pass's launch configuration equals the shape.
+// Registers are capped at 128 (at least 4 waves per SIMD, 16 per CU).
+// ---------------------------------------------------------------------------
+template <typename T, int EPT, int OP, int N, int TPL, int L, unsigned long long PLAN>
+__global__ void __launch_bounds__(L *TPL, 4) k_fixed(PassArgs p, SolveArgs sa, const double *u_in,
+                                                                     double *u_out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    PassArgs q = p;
+    q.N = N;
+    q.M = N / 2;
+    q.TPL = TPL;
+    q.L = L;
+    q.plan = PLAN;
+    // values each thread stages per tile (rows up to 16 slots of padding)
+    constexpr int CH0 = (N + 16 + TPL - 1) / TPL;
+    constexpr int CH = CH0 < StageChunk<T>::value ? CH0 : StageChunk<T>::value;
+    int a_tile, b;
+    tile_of(q, a_tile, b);
+    if constexpr (OP == B_ROWS) body_axis<T, T, T, EPT, true, CH>(q, a_tile, b, smem);
+    if constexpr (OP == B_COLS) body_axis<T, T, T, EPT, false, CH>(q, a_tile, b, smem);
+    if constexpr (OP == B_XFWD) body_xfwd<T, EPT, CH>(q, sa, a_tile, b, smem);
+    if constexpr (OP == B_ZMID) body_zmid<T, EPT, CH>(q, sa, a_tile, b, smem);
+    if constexpr (OP == B_XRES) body_xres<T, EPT, CH>(q, sa, u_in, a_tile, b, smem);
+    if constexpr (OP == B_XUPD) body_xupd<T, EPT, CH>(q, u_in, u_out, a_tile, b, smem);
 }
 
 #define CHZ_EPT_SWITCH(ept, EXPR_TEMPLATE)  \

@@ -109,6 +109,17 @@ const void *periodic_copy_kernel();
 const void *batch_kernel_f64(int op, int ept);
 const void *batch_kernel_f32(int op, int ept);
 const void *batch_periodic_kernel();
+// a compiled fixed-shape pass kernel: (precision, op, N, elements/thread, threads/line, lines, plan)
+struct FixedEntry {
+    int prec, op, N, ept, tpl, lines;
+    unsigned long long plan;
+    const void *fn;
+};
+// fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if none
+// (CHZ_NO_FIXED in the environment disables them)
+const void *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+hipError_t launch_fixed(const void *fn, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out, int prec,
+                        int nb, hipStream_t s);
 
 // Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
 // FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.

@@ -289,17 +289,17 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
 
 // Loads in flight per thread while staging a tile (the loads of a chunk are
 // all issued before the first LDS write, so a thread waits once per chunk
-// rather than once per element).
+// rather than once per element); fixed-shape kernels size it to the tile.
 template <typename T> struct StageChunk { static constexpr int value = sizeof(T) == 8 ? 24 : 32; };
 
 // Load the tile into LDS.  ROWS: lines are consecutive rows of a row-major
 // volume (x-pass), the tile is one contiguous run of memory.  !ROWS: lines
 // are L consecutive x-columns (y/z passes), slot stride s_sp.
-template <typename T, bool ROWS, typename SRC>
+template <typename T, bool ROWS, typename SRC, int CHX = 0>
 __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, const SRC *src)
 {
-    constexpr int CH = StageChunk<T>::value;
-    const int t = threadIdx.x, nt = blockDim.x;
+    constexpr int CH = CHX ? CHX : StageChunk<T>::value;
+    const int t = threadIdx.x, nt = p.L * p.TPL;
     const int NS = p.N + 2;
     const int L = p.L;
     T *rbuf = reinterpret_cast<T *>(tl.buf);
@@ -362,11 +362,11 @@ __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, 
     }
 }
 
-template <typename T, bool ROWS, typename DST>
+template <typename T, bool ROWS, typename DST, int CHX = 0>
 __device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, DST *dst)
 {
-    constexpr int CH = StageChunk<T>::value;
-    const int t = threadIdx.x, nt = blockDim.x;
+    constexpr int CH = CHX ? CHX : StageChunk<T>::value;
+    const int t = threadIdx.x, nt = p.L * p.TPL;
     const T sc = (T)p.scale;
     const T *rbuf = reinterpret_cast<const T *>(tl.buf);
     if (ROWS) {
@@ -436,9 +436,9 @@ __device__ __forceinline__ void tile_of(const PassArgs &p, int &a_tile, int &b)
 
 // Copy the M-point FFT twiddles of the pass from global memory into LDS.
 template <typename V>
-__device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M)
+__device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M, int nt)
 {
-    for (int i = threadIdx.x; i < M; i += blockDim.x) twl[i] = gtw[i];
+    for (int i = threadIdx.x; i < M; i += nt) twl[i] = gtw[i];
 }
 
 // Shared-memory carve-up of a pass: tile points, FFT twiddles, reduction scratch.
