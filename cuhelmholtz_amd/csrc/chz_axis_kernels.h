@@ -294,30 +294,30 @@ __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, 
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     {
-        constexpr int UC = 16;   // updates in flight per thread
-        const int t = threadIdx.x, nt = p.L * p.TPL;
-        const int RL = p.d_rowlen;
-        const int total = nlines * RL;
-        const long long base = (long long)a_tile * p.d_sa + (long long)k * p.d_sb;
-        const T *rbuf = reinterpret_cast<const T *>(tl.buf);
-        const double sc = p.scale;
-        for (int i0 = t; i0 < total; i0 += UC * nt) {
-            double v[UC];
-            bool w[UC];
+        // thread (row r, lane l) updates slots st_lo + l, st_lo + l + TPL, ... of row r
+        constexpr int UC = CH ? CH : 16;   // updates in flight per thread
+        const int TPL = p.TPL;
+        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        if (r < nlines) {
+            const long long rowb = (long long)(a_tile + r) * p.d_sa + (long long)k * p.d_sb;
+            const int lo = p.st_lo, hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
+            const T *rbuf = reinterpret_cast<const T *>(tl.buf);
+            const double sc = p.scale;
+            for (int s0 = lo + l; s0 <= hi; s0 += UC * TPL) {
+                double v[UC];
 #pragma unroll
-            for (int i = 0; i < UC; i++) {
-                const int idx = i0 + i * nt;
-                const int r = idx / RL, sl = idx - (idx / RL) * RL;
-                w[i] = idx < total && sl >= p.st_lo && sl <= p.st_hi;
-                v[i] = 0.0;
-                if (w[i]) {
-                    v[i] = (double)rbuf[(size_t)((sl >> 1) * tl.LS + r) * 2 + (sl & 1)] * sc;
-                    if (Uin) v[i] += Uin[base + idx];
+                for (int i = 0; i < UC; i++) {
+                    const int s = s0 + i * TPL;
+                    const int sc_ = s <= hi ? s : hi;
+                    v[i] = (double)rbuf[(size_t)((sc_ >> 1) * tl.LS + r) * 2 + (sc_ & 1)] * sc;
+                    if (Uin) v[i] += Uin[rowb + sc_];
+                }
+#pragma unroll
+                for (int i = 0; i < UC; i++) {
+                    const int s = s0 + i * TPL;
+                    if (s <= hi) out[rowb + s] = v[i];
                 }
             }
-#pragma unroll
-            for (int i = 0; i < UC; i++)
-                if (w[i]) out[base + i0 + i * nt] = v[i];
         }
     }
 }

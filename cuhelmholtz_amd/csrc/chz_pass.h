@@ -305,42 +305,34 @@ __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, 
     T *rbuf = reinterpret_cast<T *>(tl.buf);
     const int LS = tl.LS;
     if (ROWS) {
+        // thread (row r, lane l) reads slots l, l + TPL, ... of row r: consecutive
+        // lanes read consecutive doubles; every slot s < NS of every tile row is
+        // written (0 outside the load window, beyond the row or the tile's lines).
+        // The loads are unconditional at a clamped, valid index (the load window
+        // lies inside the row) and masked afterwards.
+        const int TPL = p.TPL;
+        const int r = t / TPL, l = t - (t / TPL) * TPL;
         const int RL = p.s_rowlen;
-        const int total = nlines * RL;
-        const SRC *base = src + lbase;
-        // incremental (row, slot) of idx = t + j*nt
-        int r = t / RL, s = t - (t / RL) * RL;
-        const int dq = nt / RL, dr = nt - (nt / RL) * RL;
-        for (int i0 = t; i0 < total; i0 += CH * nt) {
+        const bool rok = r < nlines;
+        const SRC *rowp = src + lbase + (long long)(rok ? r : 0) * p.s_sa;
+        const int lo = p.in_lo, hi = p.in_hi < RL - 1 ? p.in_hi : RL - 1;
+        for (int s0 = l; s0 < NS; s0 += CH * TPL) {
             T v[CH];
-            int rs[CH], ss[CH];
 #pragma unroll
             for (int i = 0; i < CH; i++) {
-                const int idx = i0 + i * nt;
-                rs[i] = r;
-                ss[i] = s;
-                v[i] = 0;
-                if (idx < total && s < NS && s >= p.in_lo && s <= p.in_hi) v[i] = (T)base[idx];
-                s += dr; r += dq;
-                if (s >= RL) { s -= RL; r++; }
+                const int s = s0 + i * TPL;
+                const bool ok = rok && s >= lo && s <= hi;
+                v[i] = (T)rowp[ok ? s : lo];
+                if (!ok) v[i] = 0;
             }
 #pragma unroll
             for (int i = 0; i < CH; i++) {
-                if (i0 + i * nt < total && ss[i] < NS) rbuf[(size_t)((ss[i] >> 1) * LS + rs[i]) * 2 + (ss[i] & 1)] = v[i];
+                const int s = s0 + i * TPL;
+                if (s < NS) rbuf[(size_t)((s >> 1) * LS + r) * 2 + (s & 1)] = v[i];
             }
         }
-        // slots beyond the source row, and lines beyond the tile end, are 0
-        if (RL < NS) {
-            const int extra = NS - RL;
-            for (int idx = t; idx < nlines * extra; idx += nt) {
-                const int rr = idx / extra, sx = RL + idx - (idx / extra) * extra;
-                rbuf[(size_t)((sx >> 1) * LS + rr) * 2 + (sx & 1)] = 0;
-            }
-        }
-        for (int idx = t; idx < (L - nlines) * NS; idx += nt) {
-            const int rr = nlines + idx / NS, sx = idx - (idx / NS) * NS;
-            rbuf[(size_t)((sx >> 1) * LS + rr) * 2 + (sx & 1)] = 0;
-        }
+        (void)nt;
+        (void)L;
     } else {
         const int c = t % L, u0 = t / L, step = nt / L;
         const bool ok = c < nlines;
@@ -370,26 +362,28 @@ __device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl,
     const T sc = (T)p.scale;
     const T *rbuf = reinterpret_cast<const T *>(tl.buf);
     if (ROWS) {
-        const int RL = p.d_rowlen;
-        const int total = nlines * RL;
-        DST *base = dst + lbase;
-        int r = t / RL, s = t - (t / RL) * RL;
-        const int dq = nt / RL, dr = nt - (nt / RL) * RL;
-        for (int i0 = t; i0 < total; i0 += CH * nt) {
+        // thread (row r, lane l) writes slots l, l + TPL, ... of row r in the store window
+        const int TPL = p.TPL;
+        const int r = t / TPL, l = t - (t / TPL) * TPL;
+        if (r >= nlines) return;
+        DST *rowp = dst + lbase + (long long)r * p.d_sa;
+        const int lo = p.st_lo, hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
+        for (int s0 = lo + l; s0 <= hi; s0 += CH * TPL) {
             T v[CH];
-            bool w[CH];
 #pragma unroll
             for (int i = 0; i < CH; i++) {
-                w[i] = i0 + i * nt < total && s >= p.st_lo && s <= p.st_hi;
-                v[i] = 0;
-                if (w[i] && s >= p.out_lo && s <= p.out_hi) v[i] = rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] * sc;
-                s += dr; r += dq;
-                if (s >= RL) { s -= RL; r++; }
+                const int s = s0 + i * TPL;
+                const int sc_ = s <= hi ? s : hi;
+                v[i] = rbuf[(size_t)((sc_ >> 1) * tl.LS + r) * 2 + (sc_ & 1)] * sc;
+                if (s < p.out_lo || s > p.out_hi) v[i] = 0;
             }
 #pragma unroll
-            for (int i = 0; i < CH; i++)
-                if (w[i]) base[i0 + i * nt] = (DST)v[i];
+            for (int i = 0; i < CH; i++) {
+                const int s = s0 + i * TPL;
+                if (s <= hi) rowp[s] = (DST)v[i];
+            }
         }
+        (void)nt;
     } else {
         const int L = p.L;
         const int c = t % L, u0 = t / L, step = nt / L;
