@@ -116,6 +116,38 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xfwd(PassArgs p, Solv
     body_xfwd<T, EPT>(p, sa, a_tile, k, smem);
 }
 
+// 1/d: hardware reciprocal estimate refined by two Newton steps
+// r <- r + r (1 - d r) (about 1 ulp; the divide of the z pass)
+__device__ __forceinline__ double recip(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    return r;
+}
+
+// the divide of the fused z pass on the tile of z lines (x = a_tile + c, y)
+template <typename T>
+__device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &sa, const Tile<T> &tl, int u, int nlines,
+                                            int a_tile, int y)
+{
+    const int c = tl.c;
+    {
+        const int x = a_tile + c;
+        const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
+        const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
+        const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+#pragma unroll 4
+        for (int s = p.out_lo + u; s <= p.out_hi; s += p.TPL) {
+            T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
+            const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
+            const double r = m0 ? 0.0 : recip(exy + ez[s] + sa.lambda);
+            v = (T)((double)v * r);
+        }
+    }
+}
+
 // Solve pass 3: z lines (x, y) of the spectral box: forward z transform,
 // multiply by 1/(eig_x + eig_y + eig_z + lambda) (the reference's
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
@@ -135,20 +167,7 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     load_tile<T, false, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    {
-        const int x = a_tile + c;
-        const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
-        const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
-        const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
-        T *rbuf = reinterpret_cast<T *>(tl.buf);
-#pragma unroll 4
-        for (int s = p.out_lo + u; s <= p.out_hi; s += p.TPL) {
-            T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
-            const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
-            const double r = m0 ? 0.0 : 1.0 / (exy + ez[s] + sa.lambda);
-            v = (T)((double)v * r);
-        }
-    }
+    zmid_divide<T>(p, sa, tl, u, nlines, a_tile, y);
     __syncthreads();
     line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     PassArgs q = p;
@@ -240,24 +259,101 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
     const int nlines = min(L, p.a0 + p.na - a_tile);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
     {
-        constexpr int RC = 8;   // residuals in flight per thread
-        const int t = threadIdx.x, nt = p.L * p.TPL, NS = p.N + 2;
+        // thread (row r, lane l) forms the residual at slots l, l + TPL, ... of
+        // row (j, k) = (a_tile + r, k).  Everything that depends only on the row
+        // -- the y / z neighbour rows of U with their boundary rules and the
+        // y / z / x boundary folds -- is resolved once per thread; per slot only
+        // the x direction remains.
+        constexpr int RC = 4;   // slots in flight per thread (7 loads each)
+        const int TPL = p.TPL;
+        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        const int NS = p.N + 2;
         T *rbuf = reinterpret_cast<T *>(tl.buf);
-        const int total = L * NS;
-        for (int i0 = t; i0 < total; i0 += RC * nt) {
+        const bool rok = r < nlines;
+        const int j = a_tile + (rok ? r : 0);
+        const long long fx = sa.NX + 1, fy = sa.NY + 1, fxy = fx * fy;
+        const long long c0 = j * fx + k * fxy;
+        const double *frow = sa.f + c0, *urow = U + c0;
+        // neighbour rows along y (d = 1) and z (d = 2): offsets from the row and 0/1 weights
+        long long dm[2], dp[2];
+        double wm[2], wp[2];
+        const int q[2] = {j, k};
+        const long long st[2] = {fx, fxy};
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const int lo = sa.s[d + 1], hi = sa.e[d + 1], bc = sa.bc[d + 1];
+            dm[d] = 0; dp[d] = 0; wm[d] = 0.0; wp[d] = 0.0;
+            if (q[d] > lo) { dm[d] = -st[d]; wm[d] = 1.0; }
+            else if (bc == 0) { dm[d] = (long long)(hi - lo) * st[d]; wm[d] = 1.0; }
+            else if ((bc == 3 || bc == 4) && hi > lo) { dm[d] = st[d]; wm[d] = 1.0; }
+            if (q[d] < hi) { dp[d] = st[d]; wp[d] = 1.0; }
+            else if (bc == 0) { dp[d] = -(long long)(hi - lo) * st[d]; wp[d] = 1.0; }
+            else if ((bc == 2 || bc == 3) && hi > lo) { dp[d] = -st[d]; wp[d] = 1.0; }
+        }
+        // y / z folds of this row: up to 4 (row pointer, coefficient) pairs read at slot s
+        const double *fp[4] = {frow, frow, frow, frow};
+        double fc[4] = {0.0, 0.0, 0.0, 0.0};
+        {
+            const int by = sa.bc[1], bz = sa.bc[2];
+            if (by != 0 && j == sa.s[1]) {
+                if (by == 1 || by == 2) { fp[0] = sa.f + k * fxy; fc[0] = -sa.ih2[1]; }
+                else { fp[0] = sa.bcb + k * fx; fc[0] = 2 * sa.ih[1]; }
+            }
+            if (by != 0 && j == sa.e[1]) {
+                if (by == 1 || by == 4) { fp[1] = sa.f + sa.NY * fx + k * fxy; fc[1] = -sa.ih2[1]; }
+                else { fp[1] = sa.bct + k * fx; fc[1] = -2 * sa.ih[1]; }
+            }
+            if (bz != 0 && k == sa.s[2]) {
+                if (bz == 1 || bz == 2) { fp[2] = sa.f + j * fx; fc[2] = -sa.ih2[2]; }
+                else { fp[2] = sa.bcf + j * fx; fc[2] = 2 * sa.ih[2]; }
+            }
+            if (bz != 0 && k == sa.e[2]) {
+                if (bz == 1 || bz == 4) { fp[3] = sa.f + j * fx + sa.NZ * fxy; fc[3] = -sa.ih2[2]; }
+                else { fp[3] = sa.bce + j * fx; fc[3] = -2 * sa.ih[2]; }
+            }
+        }
+        const bool anyfold = fc[0] != 0.0 || fc[1] != 0.0 || fc[2] != 0.0 || fc[3] != 0.0;
+        // x folds (values fixed per row) and the x boundary rule
+        const int xs = sa.s[0], xe = sa.e[0], bx = sa.bc[0];
+        double xlo = 0.0, xhi = 0.0;
+        if (bx == 1 || bx == 2) xlo = -frow[0] * sa.ih2[0];
+        else if (bx == 3 || bx == 4) xlo = 2 * sa.bcl[k * fy + j] * sa.ih[0];
+        if (bx == 1 || bx == 4) xhi = -frow[sa.NX] * sa.ih2[0];
+        else if (bx == 2 || bx == 3) xhi = -2 * sa.bcr[k * fy + j] * sa.ih[0];
+        // x neighbours at the window ends: periodic wrap, Neumann reflection, Dirichlet 0
+        const int xm_end = bx == 0 ? xe : ((bx == 3 || bx == 4) && xe > xs ? xs + 1 : xs);
+        const double xm_w = (bx == 0 || ((bx == 3 || bx == 4) && xe > xs)) ? 1.0 : 0.0;
+        const int xp_end = bx == 0 ? xs : ((bx == 2 || bx == 3) && xe > xs ? xe - 1 : xe);
+        const double xp_w = (bx == 0 || ((bx == 2 || bx == 3) && xe > xs)) ? 1.0 : 0.0;
+        const double pert = sa.pertrb ? sa.pertrb[0] * sa.pert_scale : 0.0;
+        const double *uym = urow + dm[0], *uyp = urow + dp[0], *uzm = urow + dm[1], *uzp = urow + dp[1];
+        for (int s0 = l; s0 < NS; s0 += RC * TPL) {
             double v[RC];
 #pragma unroll
             for (int i = 0; i < RC; i++) {
-                const int idx = i0 + i * nt;
-                const int r = idx / NS, sl = idx - (idx / NS) * NS;
-                v[i] = 0.0;
-                if (idx < total && r < nlines && sl >= sa.s[0] && sl <= sa.e[0]) v[i] = residual_at(sa, U, sl, a_tile + r, k);
+                const int s = s0 + i * TPL;
+                const bool ok = rok && s >= xs && s <= xe;
+                const int sc_ = ok ? s : xs;
+                const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
+                const double u0 = urow[sc_];
+                const double um = urow[sm] * (sc_ > xs ? 1.0 : xm_w), up = urow[sp] * (sc_ < xe ? 1.0 : xp_w);
+                double acc = frow[sc_] - pert;
+                if (sc_ == xs) acc += xlo;
+                if (sc_ == xe) acc += xhi;
+                if (anyfold) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++)
+                        if (fc[m] != 0.0) acc += fc[m] * fp[m][sc_];
+                }
+                double au = sa.lambda * u0 + (um - 2 * u0 + up) * sa.ih2[0];
+                au += (wm[0] * uym[sc_] - 2 * u0 + wp[0] * uyp[sc_]) * sa.ih2[1];
+                au += (wm[1] * uzm[sc_] - 2 * u0 + wp[1] * uzp[sc_]) * sa.ih2[2];
+                v[i] = ok ? acc - au : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < RC; i++) {
-                const int idx = i0 + i * nt;
-                const int r = idx / NS, sl = idx - (idx / NS) * NS;
-                if (idx < total) rbuf[(size_t)((sl >> 1) * tl.LS + r) * 2 + (sl & 1)] = (T)v[i];
+                const int s = s0 + i * TPL;
+                if (s < NS) rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] = (T)v[i];
             }
         }
     }

@@ -8,7 +8,7 @@ namespace chz {
 const FixedEntry *fixed_table_f64(int *n);
 const FixedEntry *fixed_table_f32(int *n);
 
-const void *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
+const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
 {
     static const bool off = getenv("CHZ_NO_FIXED") != nullptr;
     if (off) return nullptr;
@@ -16,7 +16,7 @@ const void *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, u
     const FixedEntry *t = prec == 64 ? fixed_table_f64(&n) : fixed_table_f32(&n);
     for (int i = 0; i < n; i++) {
         const FixedEntry &e = t[i];
-        if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return e.fn;
+        if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
     }
     return nullptr;
 }

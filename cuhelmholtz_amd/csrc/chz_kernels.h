@@ -38,6 +38,7 @@ struct PassArgs {
     int out_lo, out_hi;           // valid output slots (others stored as 0)
     int L, TPL;
     int xcds;                     // XCD count for the tile swizzle (1 = dispatch order)
+    int bfast;                    // 1: consecutive tiles step through b first (neighbour planes in time)
     unsigned long long plan;
     const void *tw;
     double scale;
@@ -115,11 +116,11 @@ struct FixedEntry {
     unsigned long long plan;
     const void *fn;
 };
-// fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if none
-// (CHZ_NO_FIXED in the environment disables them)
-const void *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
-hipError_t launch_fixed(const void *fn, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out, int prec,
-                        int nb, hipStream_t s);
+// fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if
+// none (CHZ_NO_FIXED in the environment disables them)
+const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
+                        int prec, int nb, hipStream_t s);
 
 // Launch one axis pass (prec = 64 or 32; ept = elements per thread of the
 // FFT; rows = x-pass row tiles, else column tiles) over nb lines of index b.

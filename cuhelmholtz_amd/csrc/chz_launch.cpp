@@ -12,22 +12,24 @@ size_t axis_pass_lds(int prec, int M, int L, int TPL)
     return ((size_t)lds_points(M, L) + (size_t)M) * 2 * esz + (size_t)TPL * L * esz;
 }
 
-hipError_t launch_fixed(const void *fn, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out, int prec,
-                        int nb, hipStream_t s)
+hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
+                        int prec, int nb, hipStream_t s)
 {
+    const size_t lds = axis_pass_lds(prec, p.M, p.L, p.TPL);
     dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
+    const void *fn = fx->fn;
     PassArgs a = p;
     SolveArgs b{};
     if (sa) b = *sa;
     const void *x = u_in;
     void *y = u_out;
     void *args[] = {&a, &b, &x, &y};
-    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, axis_pass_lds(prec, p.M, p.L, p.TPL), s);
+    return hipLaunchKernel(fn, grid, dim3((unsigned)(p.L * p.TPL)), args, lds, s);
 }
 
 hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s)
 {
-    if (const void *fx = fixed_kernel(prec, rows ? B_ROWS : B_COLS, p.N, ept, p.TPL, p.L, p.plan))
+    if (const FixedEntry *fx = fixed_kernel(prec, rows ? B_ROWS : B_COLS, p.N, ept, p.TPL, p.L, p.plan))
         return launch_fixed(fx, p, nullptr, nullptr, nullptr, prec, nb, s);
     const void *fn = prec == 64 ? (rows ? axis_kernel_f64_rows(ept) : axis_kernel_f64_cols(ept))
                                 : (rows ? axis_kernel_f32_rows(ept) : axis_kernel_f32_cols(ept));

@@ -424,8 +424,14 @@ __device__ __forceinline__ void tile_of(const PassArgs &p, int &a_tile, int &b)
         const int xcd = w % X, local = w / X;
         logical = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
     }
-    a_tile = p.a0 + (logical % ntiles) * p.L;
-    b = p.b0 + logical / ntiles;
+    if (p.bfast) {
+        const int nb = total / ntiles;
+        a_tile = p.a0 + (logical / nb) * p.L;
+        b = p.b0 + logical % nb;
+    } else {
+        a_tile = p.a0 + (logical % ntiles) * p.L;
+        b = p.b0 + logical / ntiles;
+    }
 }
 
 // Copy the M-point FFT twiddles of the pass from global memory into LDS.

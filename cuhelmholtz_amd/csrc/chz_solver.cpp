@@ -178,6 +178,9 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
         q.nb = g.e[2] - g.s[2] + 1;
         q.d.sa = sa;
         q.d.u_in = U;
+        // the residual reads the z-neighbour rows of U: step through z first so
+        // they are the rows of the tiles running at the same time (L2 hits)
+        p.bfast = U ? 1 : 0;
         if ((rc = check_spec(q, fvol, vol, "pass 1"))) return rc;
         out.push_back(q);
     }
@@ -286,7 +289,7 @@ int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st)
     const PassArgs &p = q.d.p;
     const bool f64 = q.prec == 64;
     hipError_t e = hipErrorInvalidValue;
-    if (const void *fx = fixed_kernel(q.prec, q.op, p.N, q.ept, p.TPL, p.L, p.plan)) {
+    if (const FixedEntry *fx = fixed_kernel(q.prec, q.op, p.N, q.ept, p.TPL, p.L, p.plan)) {
         e = launch_fixed(fx, p, &q.d.sa, q.d.u_in, q.d.u_out, q.prec, q.nb, st);
         if (e != hipSuccess) return fail(ERR_DEVICE, std::string(q.name) + " pass launch: " + hipGetErrorString(e));
         return OK;
