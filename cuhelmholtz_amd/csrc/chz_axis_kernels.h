@@ -13,7 +13,7 @@ __device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long lo
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
 template <typename T>
 __device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<T> &tl, int nlines, int j0, int k, int TPL)
 {
-    const int L = tl.LS - 1;
+    const int L = tl.L;
     const int t = threadIdx.x, nt = L * TPL;
     const int c = t % L, u0 = t / L, step = nt / L;
     if (c >= nlines) return;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
@@ -148,6 +148,70 @@ __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &
     }
 }
 
+// Periodic z (R2C forward, C2R inverse): the split of the M-point FFT into
+// the N-point half spectrum X_m (the R2C unfold), the divide of X_m by the
+// eigenvalue sum of mode m, and the Z2D combine of the C2R fold all work on
+// the point pairs (k, M - k), so they run as one pass: read Z_k, Z_{M-k},
+// form X_k, X_{M-k}, scale, recombine into the (conjugated) inverse-FFT input.
+// Same arithmetic as unfold<K_R2C> -> zmid_divide -> fold<K_C2R>.
+template <int EPT, typename T>
+__device__ __forceinline__ void periodic_split_divide(const Tile<T> &tl, int u, int TPL, int M, int nlines, int a_tile,
+                                                      int y, const SolveArgs &sa,
+                                                      const typename vec2_of<T>::type *__restrict__ tw)
+{
+    typedef typename vec2_of<T>::type V;
+    constexpr int PP = EPT / 2 + 1;
+    const T h = (T)0.5;
+    const int c = tl.c, x = a_tile + c;
+    const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
+    const double exy = c < nlines ? ex[x] + ey[y] + sa.lambda : 1.0;
+    const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
+    V ca[PP], cb[PP], wk[PP];
+#pragma unroll
+    for (int i = 0; i < PP; i++) {
+        const int k = u + i * TPL;
+        wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);   // w^k, w = e^{2 pi i / N}
+    }
+#pragma unroll
+    for (int i = 0; i < PP; i++) {
+        const int k = u + i * TPL;
+        if (k <= M / 2) {
+            const V Zk = tl.pt(k), Zm = tl.pt(k == 0 ? 0 : M - k);
+            const V w = wk[i];
+            // R2C split: X_k = E + conj(w^k) O, X_{M-k} = conj(E - conj(w^k) O)
+            const V E = mk<V>(h * (Zk.x + Zm.x), h * (Zk.y - Zm.y));
+            const V O = mk<V>(h * (Zk.y + Zm.y), -h * (Zk.x - Zm.x));
+            const V P = mk<V>(w.x * O.x + w.y * O.y, w.x * O.y - w.y * O.x);
+            V Xk = mk<V>(E.x + P.x, E.y + P.y), Xm = mk<V>(E.x - P.x, -(E.y - P.y));
+            // divide by the eigenvalue sums of modes k and M - k (mode m = slots 2m, 2m + 1)
+            const double rk = (m0xy && k == 0) ? 0.0 : recip(exy + ez[2 * k]);
+            const double rm = recip(exy + ez[2 * (M - k)]);
+            Xk = mk<V>((T)((double)Xk.x * rk), (T)((double)Xk.y * rk));
+            Xm = mk<V>((T)((double)Xm.x * rm), (T)((double)Xm.y * rm));
+            if (k == 0) {   // DC and Nyquist are real
+                Xk.y = 0;
+                Xm.y = 0;
+            }
+            // C2R combine: C_k = E' + i O', E' = H_k + conj H_{M-k}, O' = (H_k - conj H_{M-k}) w^k
+            const V E2 = mk<V>(Xk.x + Xm.x, Xk.y - Xm.y);
+            const V D2 = mk<V>(Xk.x - Xm.x, Xk.y + Xm.y);
+            const V O2 = cmul(D2, w);
+            ca[i] = mk<V>(E2.x - O2.y, -(E2.y + O2.x));
+            cb[i] = mk<V>(E2.x + O2.y, E2.y - O2.x);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PP; i++) {
+        const int k = u + i * TPL;
+        if (k <= M / 2) {
+            tl.pt(k) = ca[i];
+            if (k != 0 && 2 * k != M) tl.pt(M - k) = cb[i];
+        }
+    }
+    __syncthreads();
+}
+
 // Solve pass 3: z lines (x, y) of the spectral box: forward z transform,
 // multiply by 1/(eig_x + eig_y + eig_z + lambda) (the reference's
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
@@ -159,17 +223,24 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     const int L = p.L;
     const int c = (int)threadIdx.x % L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, L + 1, c};
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), c, L};
     const int u = (int)threadIdx.x / L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const V *tw = reinterpret_cast<const V *>(p.tw);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
     load_tile<T, false, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
     __syncthreads();
-    line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    zmid_divide<T>(p, sa, tl, u, nlines, a_tile, y);
-    __syncthreads();
-    line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
+    if (p.kind == K_R2C && sa.kind2 == K_C2R) {
+        fft_forward<EPT, T>(tl, u, p.TPL, p.M, p.plan, sm.twl);
+        periodic_split_divide<EPT, T>(tl, u, p.TPL, p.M, nlines, a_tile, y, sa, tw);
+        fft_forward<EPT, T>(tl, u, p.TPL, p.M, p.plan, sm.twl);
+        unfold<K_C2R, EPT, T>(tl, u, p.TPL, p.M, p.N, tw, sm.red);
+    } else {
+        line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
+        zmid_divide<T>(p, sa, tl, u, nlines, a_tile, y);
+        __syncthreads();
+        line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
+    }
     PassArgs q = p;
     q.out_lo = sa.out2_lo;
     q.out_hi = sa.out2_hi;
@@ -253,7 +324,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
@@ -381,7 +452,7 @@ __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, 
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, L + 1, (int)threadIdx.x % L};
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
     const int u = (int)threadIdx.x / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);

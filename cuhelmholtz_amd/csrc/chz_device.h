@@ -3,9 +3,8 @@
 // One workgroup owns a tile of L lines of one axis.  A line holds the N+2
 // real slots of the padded layout (N cells, M = N/2 complex points).  In LDS
 // the tile is stored as complex points, point-major: point k of line c sits
-// at buf[k*LS + c] with LS = L + 1 (odd, so the row-wise staging of the
-// x-pass and the column-wise staging of the y/z passes are both free of bank
-// conflicts).  Real slot s of line c is component (s & 1) of point s >> 1.
+// at buf[k*LS + c] with LS = tile_ls(L, sizeof(T)) (L, plus one padding
+// point for FP32).  Real slot s of line c is component (s & 1) of point s >> 1.
 //
 // Thread t of the workgroup works on line c = t % L as sub-thread
 // u = t / L (TPL = blockDim / L sub-threads per line), so consecutive lanes
@@ -193,8 +192,9 @@ template <> struct Dft<16> : DftPQ<4, 4> {};
 template <typename T> struct Tile {
     typedef typename vec2_of<T>::type V;
     V *buf;     // (M+1) * LS points
-    int LS;     // point stride (L + 1)
+    int LS;     // point stride (tile_ls(L))
     int c;      // line of this thread
+    int L;      // lines of the tile
     __device__ __forceinline__ V &pt(int k) const { return buf[k * LS + c]; }
     __device__ __forceinline__ T &re(int s) const { return reinterpret_cast<T *>(buf)[(size_t)((s >> 1) * LS + c) * 2 + (s & 1)]; }
 };

@@ -60,7 +60,11 @@ struct SolveArgs {
     double pert_scale;                                 // 1 / (sum of the 0-mode weights)
 };
 
-__host__ __device__ inline int lds_points(int M, int L) { return (M + 1) * (L + 1); }
+// Point stride of a tile of L lines in LDS: FP64 tiles are unpadded (a
+// smaller tile fits one more workgroup per CU, measured faster), FP32 tiles
+// carry one padding point per point row (measured faster).
+__host__ __device__ constexpr int tile_ls(int L, int esz) { return L + (esz == 8 ? 0 : 1); }
+__host__ __device__ inline int lds_points(int M, int L, int esz) { return (M + 1) * tile_ls(L, esz); }
 
 // Pass kernels of a solve (also the op of a batched pass).
 enum BatchOp { B_ROWS = 0, B_COLS = 1, B_XFWD = 2, B_ZMID = 3, B_XRES = 4, B_XUPD = 5 };

@@ -9,7 +9,7 @@ size_t axis_pass_lds(int prec, int M, int L, int TPL)
 {
     const size_t esz = prec == 64 ? 8 : 4;
     // tile points + FFT twiddles (complex) + reduction scratch (real)
-    return ((size_t)lds_points(M, L) + (size_t)M) * 2 * esz + (size_t)TPL * L * esz;
+    return ((size_t)lds_points(M, L, (int)esz) + (size_t)M) * 2 * esz + (size_t)TPL * L * esz;
 }
 
 hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,

@@ -70,7 +70,7 @@ __device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, i
         if constexpr (KIND == K_DCT_1) {   // x1 = sum of odd samples (dct1funcinplace.cu:86-93)
             T s = 0;
             for (int j = u; j < M; j += TPL) s += tl.re(2 * j + 1);
-            red[u * (tl.LS - 1) + tl.c] = s;
+            red[u * tl.L + tl.c] = s;
         }
         V wk[PP];
 #pragma unroll
@@ -165,7 +165,7 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
         T oa[PT], ob[PT];
         T x1 = 0;
         if constexpr (KIND == K_DCT_1) {
-            const int L = tl.LS - 1;
+            const int L = tl.L;
             for (int v = 0; v < TPL; v++) x1 += red[v * L + tl.c];
         }
         // per-n table value: (cos, sin)(n pi/2N) for DCT_2 / DST_2, (1/sin(n pi/N), 0) for DST_1 / DCT_1
@@ -450,7 +450,7 @@ template <typename T> struct Smem {
     __device__ __forceinline__ Smem(unsigned char *smem, int M, int L)
     {
         buf = reinterpret_cast<V *>(smem);
-        twl = buf + lds_points(M, L);
+        twl = buf + lds_points(M, L, (int)sizeof(T));
         red = reinterpret_cast<T *>(twl + M);
     }
 };
