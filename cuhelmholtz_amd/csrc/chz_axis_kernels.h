@@ -33,58 +33,64 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_axis_pass(PassArgs p)
     body_axis<T, SRC, DST, EPT, ROWS>(p, a_tile, b, smem);
 }
 
-// Boundary folding of the reference (cudahelmholtz/_cuda_helmholtz_.inl:70-227),
-// applied to the rows (j, k) of the unknown box just loaded into LDS:
+// The boundary folds of one row (j, k) of the unknown box (the reference's
+// folding, cudahelmholtz/_cuda_helmholtz_.inl:70-227, order x, y, z as in
+// cuda_helmholtz_solver.cu:234-285), resolved once per row: the x folds are
+// two constants (first / last unknown slot); the y and z folds are up to four
+// (row pointer, coefficient) pairs read at the same slot s:
 // Dirichlet  f[first] -= f[0]/h^2,   f[last] -= f[N]/h^2
 // Neumann    f[first] += 2 g_l / h,  f[last] -= 2 g_r / h
-// in the reference's order x, y, z (cuda_helmholtz_solver.cu:234-285).
-template <typename T>
-__device__ __forceinline__ void fold_boundaries(const SolveArgs &sa, const Tile<T> &tl, int nlines, int j0, int k, int TPL)
-{
-    const int L = tl.L;
-    const int t = threadIdx.x, nt = L * TPL;
-    const int c = t % L, u0 = t / L, step = nt / L;
-    if (c >= nlines) return;
-    const int j = j0 + c;
-    const long long fx = sa.NX + 1, fy = sa.NY + 1;
-    const double *row = sa.f + (long long)j * fx + (long long)k * fx * fy;
-    T *rbuf = reinterpret_cast<T *>(tl.buf);
-    const int xs = sa.s[0], xe = sa.e[0];
-    const bool ylo = sa.bc[1] != 0 && j == sa.s[1], yhi = sa.bc[1] != 0 && j == sa.e[1];
-    const bool zlo = sa.bc[2] != 0 && k == sa.s[2], zhi = sa.bc[2] != 0 && k == sa.e[2];
-    const bool xb = sa.bc[0] != 0;
-    if (!(xb || ylo || yhi || zlo || zhi)) return;
-    for (int s = xs + u0; s <= xe; s += step) {
-        if (!(xb && (s == xs || s == xe)) && !(ylo || yhi || zlo || zhi)) continue;
-        T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
-        double acc = (double)v;
-        if (xb && s == xs) {
-            if (sa.bc[0] == 1 || sa.bc[0] == 2) acc -= row[0] * sa.ih2[0];
-            else acc += 2 * sa.bcl[(long long)k * fy + j] * sa.ih[0];
+struct RowFolds {
+    const double *fp[4];
+    double fc[4];
+    double xlo, xhi;
+    bool any;
+    __device__ __forceinline__ RowFolds(const SolveArgs &sa, int j, int k)
+    {
+        const long long fx = sa.NX + 1, fy = sa.NY + 1, fxy = fx * fy;
+        const double *frow = sa.f + j * fx + k * fxy;
+        for (int m = 0; m < 4; m++) {
+            fp[m] = frow;
+            fc[m] = 0.0;
         }
-        if (xb && s == xe) {
-            if (sa.bc[0] == 1 || sa.bc[0] == 4) acc -= row[sa.NX] * sa.ih2[0];
-            else acc -= 2 * sa.bcr[(long long)k * fy + j] * sa.ih[0];
+        const int by = sa.bc[1], bz = sa.bc[2], bx = sa.bc[0];
+        if (by != 0 && j == sa.s[1]) {
+            if (by == 1 || by == 2) { fp[0] = sa.f + k * fxy; fc[0] = -sa.ih2[1]; }
+            else { fp[0] = sa.bcb + k * fx; fc[0] = 2 * sa.ih[1]; }
         }
-        if (ylo) {
-            if (sa.bc[1] == 1 || sa.bc[1] == 2) acc -= sa.f[s + (long long)k * fx * fy] * sa.ih2[1];
-            else acc += 2 * sa.bcb[(long long)k * fx + s] * sa.ih[1];
+        if (by != 0 && j == sa.e[1]) {
+            if (by == 1 || by == 4) { fp[1] = sa.f + sa.NY * fx + k * fxy; fc[1] = -sa.ih2[1]; }
+            else { fp[1] = sa.bct + k * fx; fc[1] = -2 * sa.ih[1]; }
         }
-        if (yhi) {
-            if (sa.bc[1] == 1 || sa.bc[1] == 4) acc -= sa.f[s + (long long)sa.NY * fx + (long long)k * fx * fy] * sa.ih2[1];
-            else acc -= 2 * sa.bct[(long long)k * fx + s] * sa.ih[1];
+        if (bz != 0 && k == sa.s[2]) {
+            if (bz == 1 || bz == 2) { fp[2] = sa.f + j * fx; fc[2] = -sa.ih2[2]; }
+            else { fp[2] = sa.bcf + j * fx; fc[2] = 2 * sa.ih[2]; }
         }
-        if (zlo) {
-            if (sa.bc[2] == 1 || sa.bc[2] == 2) acc -= sa.f[s + (long long)j * fx] * sa.ih2[2];
-            else acc += 2 * sa.bcf[(long long)j * fx + s] * sa.ih[2];
+        if (bz != 0 && k == sa.e[2]) {
+            if (bz == 1 || bz == 4) { fp[3] = sa.f + j * fx + sa.NZ * fxy; fc[3] = -sa.ih2[2]; }
+            else { fp[3] = sa.bce + j * fx; fc[3] = -2 * sa.ih[2]; }
         }
-        if (zhi) {
-            if (sa.bc[2] == 1 || sa.bc[2] == 4) acc -= sa.f[s + (long long)j * fx + (long long)sa.NZ * fx * fy] * sa.ih2[2];
-            else acc -= 2 * sa.bce[(long long)j * fx + s] * sa.ih[2];
-        }
-        v = (T)acc;
+        any = fc[0] != 0.0 || fc[1] != 0.0 || fc[2] != 0.0 || fc[3] != 0.0;
+        xlo = 0.0;
+        xhi = 0.0;
+        if (bx == 1 || bx == 2) xlo = -frow[0] * sa.ih2[0];
+        else if (bx == 3 || bx == 4) xlo = 2 * sa.bcl[k * fy + j] * sa.ih[0];
+        if (bx == 1 || bx == 4) xhi = -frow[sa.NX] * sa.ih2[0];
+        else if (bx == 2 || bx == 3) xhi = -2 * sa.bcr[k * fy + j] * sa.ih[0];
     }
-}
+    // the folded rhs at slot s (xs, xe: the unknown x window) from the raw value v = f[row + s]
+    __device__ __forceinline__ double apply(double v, int s, int xs, int xe) const
+    {
+        if (s == xs) v += xlo;
+        if (s == xe) v += xhi;
+        if (any) {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                if (fc[m] != 0.0) v += fc[m] * fp[m][s];
+        }
+        return v;
+    }
+};
 
 // Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
 // folds, forward x transform, store into the padded spectral workspace.
@@ -99,9 +105,51 @@ __device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
     stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
-    load_tile<T, true, double, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const double *>(p.src));
-    __syncthreads();
-    fold_boundaries<T>(sa, tl, nlines, a_tile, k, p.TPL);
+    {
+        // thread (row r, lane l) stages slots l, l + TPL, ... of row (a_tile + r, k)
+        // of f with the boundary folds applied (0 outside the unknown x window)
+        constexpr int FC = CH ? CH : StageChunk<T>::value;
+        const int TPL = p.TPL, NS = p.N + 2;
+        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        const bool rok = r < nlines;
+        const int j = a_tile + (rok ? r : 0);
+        const RowFolds rf(sa, j, k);
+        const double *frow = sa.f + (long long)j * (sa.NX + 1) + (long long)k * (sa.NX + 1) * (sa.NY + 1);
+        const int xs = sa.s[0], xe = sa.e[0];
+        T *rbuf = reinterpret_cast<T *>(tl.buf);
+        for (int s0 = l; s0 < NS; s0 += FC * TPL) {
+            // all raw loads first (one batch), then the folds: the y / z folds
+            // exist on boundary rows only, the x folds at the window ends
+            double raw[FC];
+#pragma unroll
+            for (int i = 0; i < FC; i++) {
+                const int s = s0 + i * TPL;
+                raw[i] = frow[(rok && s >= xs && s <= xe) ? s : xs];
+            }
+            if (rf.any) {
+#pragma unroll
+                for (int i = 0; i < FC; i++) {
+                    const int s = s0 + i * TPL, sc_ = (s >= xs && s <= xe) ? s : xs;
+#pragma unroll
+                    for (int m = 0; m < 4; m++)
+                        if (rf.fc[m] != 0.0) raw[i] += rf.fc[m] * rf.fp[m][sc_];
+                }
+            }
+            T v[FC];
+#pragma unroll
+            for (int i = 0; i < FC; i++) {
+                const int s = s0 + i * TPL;
+                const bool ok = rok && s >= xs && s <= xe;
+                const double x = raw[i] + (s == xs ? rf.xlo : 0.0) + (s == xe ? rf.xhi : 0.0);
+                v[i] = ok ? (T)x : (T)0;
+            }
+#pragma unroll
+            for (int i = 0; i < FC; i++) {
+                const int s = s0 + i * TPL;
+                if (s < NS) rbuf[(size_t)((s >> 1) * tl.LS + r) * 2 + (s & 1)] = v[i];
+            }
+        }
+    }
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
@@ -257,7 +305,7 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_zmid(PassArgs p, Solv
 }
 
 // FP64 7-point residual r = f_fold - (Delta_h U + lambda U) at unknown node
-// (s, j, k): the boundary folds of f (as in fold_boundaries) and the
+// (s, j, k): the boundary folds of f (as in RowFolds) and the
 // homogeneous operator (Dirichlet neighbours 0, Neumann ghost u_{-1} = u_1,
 // periodic wrap), the discrete system of cuda_helmholtz_solver.
 // U == nullptr: the folded rhs alone.  In the singular case (sa.pertrb set)
@@ -361,36 +409,8 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
             else if (bc == 0) { dp[d] = -(long long)(hi - lo) * st[d]; wp[d] = 1.0; }
             else if ((bc == 2 || bc == 3) && hi > lo) { dp[d] = -st[d]; wp[d] = 1.0; }
         }
-        // y / z folds of this row: up to 4 (row pointer, coefficient) pairs read at slot s
-        const double *fp[4] = {frow, frow, frow, frow};
-        double fc[4] = {0.0, 0.0, 0.0, 0.0};
-        {
-            const int by = sa.bc[1], bz = sa.bc[2];
-            if (by != 0 && j == sa.s[1]) {
-                if (by == 1 || by == 2) { fp[0] = sa.f + k * fxy; fc[0] = -sa.ih2[1]; }
-                else { fp[0] = sa.bcb + k * fx; fc[0] = 2 * sa.ih[1]; }
-            }
-            if (by != 0 && j == sa.e[1]) {
-                if (by == 1 || by == 4) { fp[1] = sa.f + sa.NY * fx + k * fxy; fc[1] = -sa.ih2[1]; }
-                else { fp[1] = sa.bct + k * fx; fc[1] = -2 * sa.ih[1]; }
-            }
-            if (bz != 0 && k == sa.s[2]) {
-                if (bz == 1 || bz == 2) { fp[2] = sa.f + j * fx; fc[2] = -sa.ih2[2]; }
-                else { fp[2] = sa.bcf + j * fx; fc[2] = 2 * sa.ih[2]; }
-            }
-            if (bz != 0 && k == sa.e[2]) {
-                if (bz == 1 || bz == 4) { fp[3] = sa.f + j * fx + sa.NZ * fxy; fc[3] = -sa.ih2[2]; }
-                else { fp[3] = sa.bce + j * fx; fc[3] = -2 * sa.ih[2]; }
-            }
-        }
-        const bool anyfold = fc[0] != 0.0 || fc[1] != 0.0 || fc[2] != 0.0 || fc[3] != 0.0;
-        // x folds (values fixed per row) and the x boundary rule
+        const RowFolds rf(sa, j, k);
         const int xs = sa.s[0], xe = sa.e[0], bx = sa.bc[0];
-        double xlo = 0.0, xhi = 0.0;
-        if (bx == 1 || bx == 2) xlo = -frow[0] * sa.ih2[0];
-        else if (bx == 3 || bx == 4) xlo = 2 * sa.bcl[k * fy + j] * sa.ih[0];
-        if (bx == 1 || bx == 4) xhi = -frow[sa.NX] * sa.ih2[0];
-        else if (bx == 2 || bx == 3) xhi = -2 * sa.bcr[k * fy + j] * sa.ih[0];
         // x neighbours at the window ends: periodic wrap, Neumann reflection, Dirichlet 0
         const int xm_end = bx == 0 ? xe : ((bx == 3 || bx == 4) && xe > xs ? xs + 1 : xs);
         const double xm_w = (bx == 0 || ((bx == 3 || bx == 4) && xe > xs)) ? 1.0 : 0.0;
@@ -408,14 +428,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
                 const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
                 const double u0 = urow[sc_];
                 const double um = urow[sm] * (sc_ > xs ? 1.0 : xm_w), up = urow[sp] * (sc_ < xe ? 1.0 : xp_w);
-                double acc = frow[sc_] - pert;
-                if (sc_ == xs) acc += xlo;
-                if (sc_ == xe) acc += xhi;
-                if (anyfold) {
-#pragma unroll
-                    for (int m = 0; m < 4; m++)
-                        if (fc[m] != 0.0) acc += fc[m] * fp[m][sc_];
-                }
+                const double acc = rf.apply(frow[sc_], sc_, xs, xe) - pert;
                 double au = sa.lambda * u0 + (um - 2 * u0 + up) * sa.ih2[0];
                 au += (wm[0] * uym[sc_] - 2 * u0 + wp[0] * uyp[sc_]) * sa.ih2[1];
                 au += (wm[1] * uzm[sc_] - 2 * u0 + wp[1] * uzp[sc_]) * sa.ih2[2];
