@@ -383,7 +383,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
         // -- the y / z neighbour rows of U with their boundary rules and the
         // y / z / x boundary folds -- is resolved once per thread; per slot only
         // the x direction remains.
-        constexpr int RC = 4;   // slots in flight per thread (7 loads each)
+        constexpr int RC = 4;   // slots in flight per thread (8 loads each)
         const int TPL = p.TPL;
         const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
         const int NS = p.N + 2;
@@ -419,19 +419,41 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
         const double pert = sa.pertrb ? sa.pertrb[0] * sa.pert_scale : 0.0;
         const double *uym = urow + dm[0], *uyp = urow + dp[0], *uzm = urow + dm[1], *uzp = urow + dp[1];
         for (int s0 = l; s0 < NS; s0 += RC * TPL) {
+            // all loads of the chunk first (one batch), then the rare y / z
+            // folds of boundary rows, then the arithmetic
+            double fv[RC], u0[RC], um[RC], up[RC], uy[RC], uz[RC];
+#pragma unroll
+            for (int i = 0; i < RC; i++) {
+                const int s = s0 + i * TPL;
+                const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
+                const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
+                fv[i] = frow[sc_];
+                u0[i] = urow[sc_];
+                um[i] = urow[sm];
+                up[i] = urow[sp];
+                uy[i] = wm[0] * uym[sc_] + wp[0] * uyp[sc_];
+                uz[i] = wm[1] * uzm[sc_] + wp[1] * uzp[sc_];
+            }
+            if (rf.any) {
+#pragma unroll
+                for (int i = 0; i < RC; i++) {
+                    const int s = s0 + i * TPL, sc_ = (s >= xs && s <= xe) ? s : xs;
+#pragma unroll
+                    for (int m = 0; m < 4; m++)
+                        if (rf.fc[m] != 0.0) fv[i] += rf.fc[m] * rf.fp[m][sc_];
+                }
+            }
             double v[RC];
 #pragma unroll
             for (int i = 0; i < RC; i++) {
                 const int s = s0 + i * TPL;
                 const bool ok = rok && s >= xs && s <= xe;
-                const int sc_ = ok ? s : xs;
-                const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
-                const double u0 = urow[sc_];
-                const double um = urow[sm] * (sc_ > xs ? 1.0 : xm_w), up = urow[sp] * (sc_ < xe ? 1.0 : xp_w);
-                const double acc = rf.apply(frow[sc_], sc_, xs, xe) - pert;
-                double au = sa.lambda * u0 + (um - 2 * u0 + up) * sa.ih2[0];
-                au += (wm[0] * uym[sc_] - 2 * u0 + wp[0] * uyp[sc_]) * sa.ih2[1];
-                au += (wm[1] * uzm[sc_] - 2 * u0 + wp[1] * uzp[sc_]) * sa.ih2[2];
+                const double acc = fv[i] + (s == xs ? rf.xlo : 0.0) + (s == xe ? rf.xhi : 0.0) - pert;
+                const double a = u0[i];
+                const double m_ = um[i] * (s > xs ? 1.0 : xm_w), p_ = up[i] * (s < xe ? 1.0 : xp_w);
+                double au = sa.lambda * a + (m_ - 2 * a + p_) * sa.ih2[0];
+                au += (uy[i] - 2 * a) * sa.ih2[1];
+                au += (uz[i] - 2 * a) * sa.ih2[2];
                 v[i] = ok ? acc - au : 0.0;
             }
 #pragma unroll
