@@ -52,27 +52,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# kernel behind each FP64 solve pass (for the HBM traffic measured by rocprofv3)
-PASS_KERNEL = {"xfwd": "chz::k_solve_xfwd<double, 8>", "yfwd": "chz::k_axis_pass<double, double, double, 8, false>",
-               "zmid": "chz::k_solve_zmid<double, 8>", "yinv": "chz::k_axis_pass<double, double, double, 8, false>",
-               "xinv": "chz::k_axis_pass<double, double, double, 8, true>"}
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel from profiles/traffic_512.txt (two
-    rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, calibrated against a copy
-    kernel of known size); None if not profiled."""
+def measured_traffic(pass_name):
+    """(HBM bytes per launch, kernel name) of the kernel behind a solve pass,
+    from profiles/traffic_512.txt (two rocprofv3 --pmc runs, FETCH_SIZE then
+    WRITE_SIZE, calibrated against a copy kernel of known size); (None, None)
+    if not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic_512.txt")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         for line in f:
             if line.startswith("#") or not line.strip():
                 continue
-            parts = line.rsplit(None, 2)
-            if parts[0].strip() == kernel:
-                return (float(parts[1]) + float(parts[2])) * 1e9
-    return None
+            parts = line.split(None, 3)
+            if parts[0] == pass_name:
+                return (float(parts[1]) + float(parts[2])) * 1e9, parts[3].strip()
+    return None, None
 
 
 def cpu_baseline(n, bc, lam):
@@ -359,6 +354,9 @@ def main():
         cpu = cpu_baseline(n, bc, lam)
         log(f"cpu baseline: {cpu}")
 
+    dom_traffic, dom_kernel = measured_traffic(dom) if n == (512, 512, 512) else (None, None)
+    if dom_kernel is None:
+        dom_kernel = f"pass {dom}"
     best_path, value, best_ms = "fp64", solves, ms
     if mixed_info and mixed_info["solves_per_s"] > solves:
         best_path, value, best_ms = "fp32+refine", mixed_info["solves_per_s"], mixed_info["ms"]
@@ -383,9 +381,9 @@ def main():
                    "transforms_per_s_fp64": 1000.0 / tms,
                    "batched": batched,
                    "adjoint": adjoint},
-        "roofline": {"bound": "hbm", "kernel": PASS_KERNEL.get(dom, dom), "pass": dom, "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "pass": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(PASS_KERNEL.get(dom, dom)) if n == (512, 512, 512) else None},
+                     "traffic": dom_traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
