@@ -8,20 +8,21 @@ namespace chz {
 
 // One fused axis pass over a tile of L lines: load -> fold -> FFT -> unfold -> store.
 template <typename T, typename SRC, typename DST, int EPT, bool ROWS, int CH = 0>
-__device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long long b, unsigned char *smem)
+__device__ __forceinline__ void body_axis(const PassArgs &p, int a_tile, long long b, unsigned char *smem, int tid = -1)
 {
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
-    const int u = (int)threadIdx.x / L;
+    const int t0 = tid < 0 ? (int)threadIdx.x : tid;
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), t0 % L, L};
+    const int u = t0 / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
-    load_tile<T, ROWS, SRC, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL, t0);
+    load_tile<T, ROWS, SRC, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + b * p.s_sb, reinterpret_cast<const SRC *>(p.src), t0);
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, ROWS, DST, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst));
+    store_tile<T, ROWS, DST, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + b * p.d_sb, reinterpret_cast<DST *>(p.dst), t0);
 }
 
 template <typename T, typename SRC, typename DST, int EPT, bool ROWS>
@@ -95,22 +96,24 @@ struct RowFolds {
 // Solve pass 1: rows (j,k) of the unknown box of f (node layout), boundary
 // folds, forward x transform, store into the padded spectral workspace.
 template <typename T, int EPT, int CH = 0>
-__device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa, int a_tile, int k, unsigned char *smem)
+__device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa, int a_tile, int k, unsigned char *smem,
+                                          int tid = -1)
 {
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
-    const int u = (int)threadIdx.x / L;
+    const int t0 = tid < 0 ? (int)threadIdx.x : tid;
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), t0 % L, L};
+    const int u = t0 / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL, t0);
     {
         // thread (row r, lane l) stages slots l, l + TPL, ... of row (a_tile + r, k)
         // of f with the boundary folds applied (0 outside the unknown x window)
         constexpr int FC = CH ? CH : StageChunk<T>::value;
         const int TPL = p.TPL, NS = p.N + 2;
-        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        const int t = t0, r = t / TPL, l = t - (t / TPL) * TPL;
         const bool rok = r < nlines;
         const int j = a_tile + (rok ? r : 0);
         const RowFolds rf(sa, j, k);
@@ -152,7 +155,7 @@ __device__ __forceinline__ void body_xfwd(const PassArgs &p, const SolveArgs &sa
     }
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst), t0);
 }
 
 template <typename T, int EPT>
@@ -265,18 +268,20 @@ __device__ __forceinline__ void periodic_split_divide(const Tile<T> &tl, int u, 
 // _divide_ / _periodic_divide_, cudahelmholtz/_cuda_helmholtz_.inl:276-421),
 // inverse z transform, all in LDS.
 template <typename T, int EPT, int CH = 0>
-__device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa, int a_tile, int y, unsigned char *smem)
+__device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa, int a_tile, int y, unsigned char *smem,
+                                          int tid = -1)
 {
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
-    const int c = (int)threadIdx.x % L;
+    const int t0 = tid < 0 ? (int)threadIdx.x : tid;
+    const int c = t0 % L;
     const Smem<T> sm(smem, p.M, L);
     const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), c, L};
-    const int u = (int)threadIdx.x / L;
+    const int u = t0 / L;
     const int nlines = min(L, p.a0 + p.na - a_tile);
     const V *tw = reinterpret_cast<const V *>(p.tw);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
-    load_tile<T, false, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL, t0);
+    load_tile<T, false, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)y * p.s_sb, reinterpret_cast<const T *>(p.src), t0);
     __syncthreads();
     if (p.kind == K_R2C && sa.kind2 == K_C2R) {
         fft_forward<EPT, T>(tl, u, p.TPL, p.M, p.plan, sm.twl);
@@ -292,7 +297,7 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     PassArgs q = p;
     q.out_lo = sa.out2_lo;
     q.out_hi = sa.out2_hi;
-    store_tile<T, false, T, CH>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, false, T, CH>(q, tl, nlines, (long long)a_tile * p.d_sa + (long long)y * p.d_sb, reinterpret_cast<T *>(p.dst), t0);
 }
 
 template <typename T, int EPT>
@@ -367,16 +372,17 @@ __device__ __forceinline__ double residual_at(const SolveArgs &sa, const double 
 // the current solution U, rounded to T, forward x transform -> workspace.
 template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa, const double *U, int a_tile, int k,
-                                          unsigned char *smem)
+                                          unsigned char *smem, int tid = -1)
 {
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
-    const int u = (int)threadIdx.x / L;
+    const int t0 = tid < 0 ? (int)threadIdx.x : tid;
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), t0 % L, L};
+    const int u = t0 / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL, t0);
     {
         // thread (row r, lane l) forms the residual at slots l, l + TPL, ... of
         // row (j, k) = (a_tile + r, k).  Everything that depends only on the row
@@ -385,7 +391,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
         // the x direction remains.
         constexpr int RC = 4;   // slots in flight per thread (8 loads each)
         const int TPL = p.TPL;
-        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        const int t = t0, r = t / TPL, l = t - (t / TPL) * TPL;
         const int NS = p.N + 2;
         T *rbuf = reinterpret_cast<T *>(tl.buf);
         const bool rok = r < nlines;
@@ -465,7 +471,7 @@ __device__ __forceinline__ void body_xres(const PassArgs &p, const SolveArgs &sa
     }
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst));
+    store_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.d_sa + (long long)k * p.d_sb, reinterpret_cast<T *>(p.dst), t0);
 }
 
 template <typename T, int EPT>
@@ -482,24 +488,25 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_solve_xres(PassArgs p, Solv
 // (Uin == nullptr: out = ratio * delta).
 template <typename T, int EPT, int CH = 0>
 __device__ __forceinline__ void body_xupd(const PassArgs &p, const double *Uin, double *out, int a_tile, int k,
-                                          unsigned char *smem)
+                                          unsigned char *smem, int tid = -1)
 {
     typedef typename vec2_of<T>::type V;
     const int L = p.L;
     const Smem<T> sm(smem, p.M, L);
-    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), (int)threadIdx.x % L, L};
-    const int u = (int)threadIdx.x / L;
+    const int t0 = tid < 0 ? (int)threadIdx.x : tid;
+    const Tile<T> tl{sm.buf, tile_ls(L, (int)sizeof(T)), t0 % L, L};
+    const int u = t0 / L;
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const int nlines = min(L, p.a0 + p.na - a_tile);
-    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL);
-    load_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src));
+    stage_twiddles(sm.twl, tw + TwOffsets::fftw(p.N), p.M, p.L * p.TPL, t0);
+    load_tile<T, true, T, CH>(p, tl, nlines, (long long)a_tile * p.s_sa + (long long)k * p.s_sb, reinterpret_cast<const T *>(p.src), t0);
     __syncthreads();
     line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     {
         // thread (row r, lane l) updates slots st_lo + l, st_lo + l + TPL, ... of row r
         constexpr int UC = CH ? CH : 16;   // updates in flight per thread
         const int TPL = p.TPL;
-        const int t = threadIdx.x, r = t / TPL, l = t - (t / TPL) * TPL;
+        const int t = t0, r = t / TPL, l = t - (t / TPL) * TPL;
         if (r < nlines) {
             const long long rowb = (long long)(a_tile + r) * p.d_sa + (long long)k * p.d_sb;
             const int lo = p.st_lo, hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
@@ -558,11 +565,30 @@ __global__ void __launch_bounds__(256) k_periodic_copy(T *f, int NX, int NY, int
 // tiles of one problem share an L2.  The pass arguments of every problem come
 // from a descriptor table.
 // ---------------------------------------------------------------------------
-template <typename T, int EPT, int OP>
-__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_batch(const BatchItem *__restrict__ items, int nitems,
-                                                           const BatchDesc *__restrict__ descs)
+// Shape overrides of a batched pass: none (generic), or compile-time N,
+// threads per line, lines and radix plan (fixed-shape batched kernels).
+struct NoShape {
+    static constexpr int CH = 0;
+    __device__ __forceinline__ static void apply(PassArgs &) {}
+};
+template <int N, int TPL, int L, unsigned long long PLAN, typename T> struct FixShape {
+    static constexpr int CH0 = (N + 16 + TPL - 1) / TPL;
+    static constexpr int CH = CH0 < StageChunk<T>::value ? CH0 : StageChunk<T>::value;
+    __device__ __forceinline__ static void apply(PassArgs &p)
+    {
+        p.N = N;
+        p.M = N / 2;
+        p.TPL = TPL;
+        p.L = L;
+        p.plan = PLAN;
+    }
+};
+
+template <typename T, int EPT, int OP, typename S>
+__device__ __forceinline__ void batch_loop(const BatchItem *__restrict__ items, int nitems,
+                                           const BatchDesc *__restrict__ descs, unsigned char *smem)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int CH = S::CH;
     const int G = (int)gridDim.x, bid = (int)blockIdx.x;
     int w, step, end;
     if (G >= 8 && G % 8 == 0) {
@@ -576,19 +602,41 @@ __global__ void __launch_bounds__(CHZ_MAX_THREADS) k_batch(const BatchItem *__re
         end = nitems;
     }
     for (; w < end; w += step) {
+        // a thread index the compiler cannot hoist address arithmetic out of
+        // the item loop with (which would hold it in registers across items)
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
         const BatchItem it = items[w];
         const BatchDesc *d = descs + it.prob;
-        const PassArgs p = d->p;
+        PassArgs p = d->p;
+        S::apply(p);
         const int ntiles = (p.na + p.L - 1) / p.L;
         const int a_tile = p.a0 + (it.tile % ntiles) * p.L, b = p.b0 + it.tile / ntiles;
-        if constexpr (OP == B_ROWS) body_axis<T, T, T, EPT, true>(p, a_tile, b, smem);
-        if constexpr (OP == B_COLS) body_axis<T, T, T, EPT, false>(p, a_tile, b, smem);
-        if constexpr (OP == B_XFWD) body_xfwd<T, EPT>(p, d->sa, a_tile, b, smem);
-        if constexpr (OP == B_ZMID) body_zmid<T, EPT>(p, d->sa, a_tile, b, smem);
-        if constexpr (OP == B_XRES) body_xres<T, EPT>(p, d->sa, d->u_in, a_tile, b, smem);
-        if constexpr (OP == B_XUPD) body_xupd<T, EPT>(p, d->u_in, d->u_out, a_tile, b, smem);
+        if constexpr (OP == B_ROWS) body_axis<T, T, T, EPT, true, CH>(p, a_tile, b, smem, tid);
+        if constexpr (OP == B_COLS) body_axis<T, T, T, EPT, false, CH>(p, a_tile, b, smem, tid);
+        if constexpr (OP == B_XFWD) body_xfwd<T, EPT, CH>(p, d->sa, a_tile, b, smem, tid);
+        if constexpr (OP == B_ZMID) body_zmid<T, EPT, CH>(p, d->sa, a_tile, b, smem, tid);
+        if constexpr (OP == B_XRES) body_xres<T, EPT, CH>(p, d->sa, d->u_in, a_tile, b, smem, tid);
+        if constexpr (OP == B_XUPD) body_xupd<T, EPT, CH>(p, d->u_in, d->u_out, a_tile, b, smem, tid);
         __syncthreads();   // the next item reuses the tile memory
     }
+}
+
+template <typename T, int EPT, int OP>
+__global__ void __launch_bounds__(CHZ_MAX_THREADS) k_batch(const BatchItem *__restrict__ items, int nitems,
+                                                           const BatchDesc *__restrict__ descs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    batch_loop<T, EPT, OP, NoShape>(items, nitems, descs, smem);
+}
+
+// fixed-shape batched pass (one size class with a compiled launch configuration)
+template <typename T, int EPT, int OP, int N, int TPL, int L, unsigned long long PLAN>
+__global__ void __launch_bounds__(L *TPL, 4) k_batch_fixed(const BatchItem *__restrict__ items, int nitems,
+                                                           const BatchDesc *__restrict__ descs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    batch_loop<T, EPT, OP, FixShape<N, TPL, L, PLAN, T>>(items, nitems, descs, smem);
 }
 
 // periodic duplicate copies of many problems: copy blockIdx.y, face element x

@@ -265,7 +265,8 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         const Group &gr = groups[gi];
         const PassSpec &q = *gr.first;
         BatchLaunch l;
-        l.fn = q.prec == 64 ? batch_kernel_f64(q.op, q.ept) : batch_kernel_f32(q.op, q.ept);
+        l.fn = batch_fixed_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+        if (!l.fn) l.fn = q.prec == 64 ? batch_kernel_f64(q.op, q.ept) : batch_kernel_f32(q.op, q.ept);
         if (!l.fn) {
             delete plan;
             return fail(ERR_UNSUPPORTED, "no batched kernel for this pass");

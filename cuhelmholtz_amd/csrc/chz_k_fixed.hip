@@ -7,6 +7,11 @@ namespace chz {
 
 const FixedEntry *fixed_table_f64(int *n);
 const FixedEntry *fixed_table_f32(int *n);
+const FixedEntry *batch_fixed_table_64_0(int *n);
+const FixedEntry *batch_fixed_table_64_1(int *n);
+const FixedEntry *batch_fixed_table_32_0(int *n);
+const FixedEntry *batch_fixed_table_32_1(int *n);
+const FixedEntry *batch_fixed_table_32_2(int *n);
 
 const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
 {
@@ -17,6 +22,26 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
     for (int i = 0; i < n; i++) {
         const FixedEntry &e = t[i];
         if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
+    }
+    return nullptr;
+}
+
+const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
+{
+    static const bool off = getenv("CHZ_NO_FIXED") != nullptr;
+    if (off) return nullptr;
+    typedef const FixedEntry *(*Tab)(int *);
+    static const Tab t64[] = {batch_fixed_table_64_0, batch_fixed_table_64_1};
+    static const Tab t32[] = {batch_fixed_table_32_0, batch_fixed_table_32_1, batch_fixed_table_32_2};
+    const Tab *tabs = prec == 64 ? t64 : t32;
+    const int ntab = prec == 64 ? 2 : 3;
+    for (int q = 0; q < ntab; q++) {
+        int n = 0;
+        const FixedEntry *t = tabs[q](&n);
+        for (int i = 0; i < n; i++) {
+            const FixedEntry &e = t[i];
+            if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return e.fn;
+        }
     }
     return nullptr;
 }

@@ -287,6 +287,10 @@ __device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M,
 
 
 
+// The pass helpers take the thread index as an optional argument (tid < 0:
+// threadIdx.x) so a loop over tiles can hand them an index the compiler
+// cannot hoist address arithmetic out of.
+//
 // Loads in flight per thread while staging a tile (the loads of a chunk are
 // all issued before the first LDS write, so a thread waits once per chunk
 // rather than once per element); fixed-shape kernels size it to the tile.
@@ -296,10 +300,11 @@ template <typename T> struct StageChunk { static constexpr int value = sizeof(T)
 // volume (x-pass), the tile is one contiguous run of memory.  !ROWS: lines
 // are L consecutive x-columns (y/z passes), slot stride s_sp.
 template <typename T, bool ROWS, typename SRC, int CHX = 0>
-__device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, const SRC *src)
+__device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, const SRC *src,
+                                          int tid = -1)
 {
     constexpr int CH = CHX ? CHX : StageChunk<T>::value;
-    const int t = threadIdx.x, nt = p.L * p.TPL;
+    const int t = tid < 0 ? (int)threadIdx.x : tid, nt = p.L * p.TPL;
     const int NS = p.N + 2;
     const int L = p.L;
     T *rbuf = reinterpret_cast<T *>(tl.buf);
@@ -355,10 +360,11 @@ __device__ __forceinline__ void load_tile(const PassArgs &p, const Tile<T> &tl, 
 }
 
 template <typename T, bool ROWS, typename DST, int CHX = 0>
-__device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, DST *dst)
+__device__ __forceinline__ void store_tile(const PassArgs &p, const Tile<T> &tl, int nlines, long long lbase, DST *dst,
+                                           int tid = -1)
 {
     constexpr int CH = CHX ? CHX : StageChunk<T>::value;
-    const int t = threadIdx.x, nt = p.L * p.TPL;
+    const int t = tid < 0 ? (int)threadIdx.x : tid, nt = p.L * p.TPL;
     const T sc = (T)p.scale;
     const T *rbuf = reinterpret_cast<const T *>(tl.buf);
     if (ROWS) {
@@ -436,9 +442,9 @@ __device__ __forceinline__ void tile_of(const PassArgs &p, int &a_tile, int &b)
 
 // Copy the M-point FFT twiddles of the pass from global memory into LDS.
 template <typename V>
-__device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M, int nt)
+__device__ __forceinline__ void stage_twiddles(V *twl, const V *gtw, int M, int nt, int tid = -1)
 {
-    for (int i = threadIdx.x; i < M; i += nt) twl[i] = gtw[i];
+    for (int i = tid < 0 ? (int)threadIdx.x : tid; i < M; i += nt) twl[i] = gtw[i];
 }
 
 // Shared-memory carve-up of a pass: tile points, FFT twiddles, reduction scratch.
