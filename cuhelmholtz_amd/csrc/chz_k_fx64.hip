@@ -1,5 +1,6 @@
 // FP64 fixed-shape pass kernels for N = 512 (elements/thread 8, 32 threads per
 // line, 8 or 16 lines per tile): every role of the solve and the transform.
+// (16 points per thread with radix 16 x 16 measured slower in every role.)
 #include <cstdlib>
 
 #include "chz_axis_kernels.h"
