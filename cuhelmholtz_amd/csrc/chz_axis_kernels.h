@@ -205,8 +205,8 @@ __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &
 // the point pairs (k, M - k), so they run as one pass: read Z_k, Z_{M-k},
 // form X_k, X_{M-k}, scale, recombine into the (conjugated) inverse-FFT input.
 // Same arithmetic as unfold<K_R2C> -> zmid_divide -> fold<K_C2R>.
-template <int EPT, typename T>
-__device__ __forceinline__ void periodic_split_divide(const Tile<T> &tl, int u, int TPL, int M, int nlines, int a_tile,
+template <int EPT, typename T, typename TL>
+__device__ __forceinline__ void periodic_split_divide(const TL &tl, int u, int TPL, int M, int nlines, int a_tile,
                                                       int y, const SolveArgs &sa,
                                                       const typename vec2_of<T>::type *__restrict__ tw)
 {

@@ -202,11 +202,10 @@ template <typename T> struct Tile {
 // One Stockham stage of radix R over M points, in place.  Thread u of a line
 // runs the butterflies b = u + i*TPL (i < EPT/R, b < M/R), so the stage fits
 // when ceil(M/(R*TPL)) <= EPT/R (the host planner guarantees it).
-template <int R, int EPT, typename T>
-__device__ __forceinline__ void fft_stage(const Tile<T> &tl, int u, int TPL, int M, int Ns,
-                                          const typename vec2_of<T>::type *__restrict__ tw)
+template <int R, int EPT, typename TL>
+__device__ __forceinline__ void fft_stage(const TL &tl, int u, int TPL, int M, int Ns, const typename TL::V *__restrict__ tw)
 {
-    typedef typename vec2_of<T>::type V;
+    typedef typename TL::V V;
     constexpr int NB = EPT / R;
     const int MR = M / R;
     V v[NB * R];
@@ -241,8 +240,8 @@ __device__ __forceinline__ void fft_stage(const Tile<T> &tl, int u, int TPL, int
 
 // Full forward FFT: stages described by a packed radix list (one byte per
 // stage, first stage in the low byte, 0 terminates).
-template <int EPT, typename T>
-__device__ __forceinline__ void fft_forward(const Tile<T> &tl, int u, int TPL, int M, unsigned long long plan,
+template <int EPT, typename T, typename TL>
+__device__ __forceinline__ void fft_forward(const TL &tl, int u, int TPL, int M, unsigned long long plan,
                                             const typename vec2_of<T>::type *__restrict__ tw)
 {
     int Ns = 1;

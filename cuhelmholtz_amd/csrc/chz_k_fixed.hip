@@ -6,6 +6,7 @@
 namespace chz {
 
 const FixedEntry *fixed_table_f64(int *n);
+const FixedEntry *line_table_f64(int *n);
 const FixedEntry *fixed_table_f32(int *n);
 const FixedEntry *batch_fixed_table_64_0(int *n);
 const FixedEntry *batch_fixed_table_64_1(int *n);
@@ -13,11 +14,30 @@ const FixedEntry *batch_fixed_table_32_0(int *n);
 const FixedEntry *batch_fixed_table_32_1(int *n);
 const FixedEntry *batch_fixed_table_32_2(int *n);
 
-const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
+// The lean-I/O kernels check the input, store and output windows only in the
+// first and last slot chunks of a line: every window must contain 2..N-2.
+static bool line_windows_ok(const PassArgs &p, int N)
+{
+    auto in = [N](int lo, int hi) { return lo <= 2 && hi >= N - 2; };
+    return in(p.in_lo, p.in_hi) && in(p.st_lo, p.st_hi) && in(p.out_lo, p.out_hi);
+}
+
+const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
+                               const PassArgs *p)
 {
     static const bool off = getenv("CHZ_NO_FIXED") != nullptr;
     if (off) return nullptr;
+    // FP64: the lean-I/O kernels first (CHZ_NO_LINE in the environment: the
+    // plain fixed-shape kernels)
+    static const bool no_line = getenv("CHZ_NO_LINE") != nullptr;
     int n = 0;
+    if (prec == 64 && !no_line && p && line_windows_ok(*p, N)) {
+        const FixedEntry *t = line_table_f64(&n);
+        for (int i = 0; i < n; i++) {
+            const FixedEntry &e = t[i];
+            if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
+        }
+    }
     const FixedEntry *t = prec == 64 ? fixed_table_f64(&n) : fixed_table_f32(&n);
     for (int i = 0; i < n; i++) {
         const FixedEntry &e = t[i];

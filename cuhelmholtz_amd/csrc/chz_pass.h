@@ -26,8 +26,8 @@ namespace chz {
 
 // Half-complex spectrum H_k (k = 0..M) that the reference's preOp builds for
 // the Z2D-type kinds, read from the real line x.
-template <int KIND, typename T>
-__device__ __forceinline__ typename vec2_of<T>::type hval(const Tile<T> &tl, int k, int M, int N)
+template <int KIND, typename T, typename TL>
+__device__ __forceinline__ typename vec2_of<T>::type hval(const TL &tl, int k, int M, int N)
 {
     typedef typename vec2_of<T>::type V;
     const T h = (T)0.5;
@@ -55,8 +55,8 @@ __device__ __forceinline__ typename vec2_of<T>::type hval(const Tile<T> &tl, int
     }
 }
 
-template <int KIND, int EPT, typename T>
-__device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, int N,
+template <int KIND, int EPT, typename T, typename TL>
+__device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N,
                                      const typename vec2_of<T>::type *__restrict__ tw, T *red)
 {
     typedef typename vec2_of<T>::type V;
@@ -152,8 +152,8 @@ __device__ __forceinline__ void fold(const Tile<T> &tl, int u, int TPL, int M, i
 // ---------------------------------------------------------------------------
 // Unfold: FFT output points -> real output slots, in place.
 // ---------------------------------------------------------------------------
-template <int KIND, int EPT, typename T>
-__device__ __forceinline__ void unfold(const Tile<T> &tl, int u, int TPL, int M, int N,
+template <int KIND, int EPT, typename T, typename TL>
+__device__ __forceinline__ void unfold(const TL &tl, int u, int TPL, int M, int N,
                                        const typename vec2_of<T>::type *__restrict__ tw, const T *red)
 {
     typedef typename vec2_of<T>::type V;
@@ -462,8 +462,8 @@ template <typename T> struct Smem {
 };
 
 // One line transform of the whole tile (fold, FFT, unfold), data in LDS.
-template <int KIND, int EPT, typename T>
-__device__ __forceinline__ void line_transform_k(const Tile<T> &tl, int u, int TPL, int M, int N,
+template <int KIND, int EPT, typename T, typename TL>
+__device__ __forceinline__ void line_transform_k(const TL &tl, int u, int TPL, int M, int N,
                                                  unsigned long long plan, const typename vec2_of<T>::type *tw,
                                                  const typename vec2_of<T>::type *twl, T *red)
 {
@@ -474,8 +474,8 @@ __device__ __forceinline__ void line_transform_k(const Tile<T> &tl, int u, int T
 
 // Runtime kind -> one specialised path (the branch is uniform over the workgroup).
 // tw: the pass's twiddle table in global memory; twl: its FFT part staged in LDS.
-template <int EPT, typename T>
-__device__ __forceinline__ void line_transform(int kind, const Tile<T> &tl, int u, int TPL, int M, int N,
+template <int EPT, typename T, typename TL>
+__device__ __forceinline__ void line_transform(int kind, const TL &tl, int u, int TPL, int M, int N,
                                                unsigned long long plan, const typename vec2_of<T>::type *tw,
                                                const typename vec2_of<T>::type *twl, T *red)
 {

@@ -73,6 +73,27 @@ def test_all_bc_vs_oracle(n, dom):
         assert rel(got, want) < TOL, bc
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_long_axis_every_bc_vs_oracle(axis):
+    """512 cells along one axis (the fixed-shape pass kernels of that axis's
+    role: x rows, y columns, fused z pass) with every code on it, partial
+    tiles on the short axes, against the oracle."""
+    rng = np.random.default_rng(40 + axis)
+    n = [6, 8, 10]
+    n[axis] = 512
+    n = tuple(n)
+    for code in range(5):
+        bc = [int(v) for v in rng.integers(0, 5, 3)]
+        bc[axis] = code
+        bc = tuple(bc)
+        for lam in (-2.3, 0.0):
+            f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+            bcs = faces(rng, n, bc)
+            got = gpu_solve(f, n, bc, lam, bcs)
+            want = O.helmholtz(f, n, bc, lam, bcs)
+            assert rel(got, want) < TOL, (n, bc, lam)
+
+
 @pytest.mark.parametrize("bc", [(0, 0, 0), (3, 3, 3), (0, 3, 0), (3, 0, 3)])
 def test_singular_zero_mode(bc):
     rng = np.random.default_rng(2)
