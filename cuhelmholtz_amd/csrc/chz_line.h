@@ -2,9 +2,9 @@
 //
 // The fold, FFT and unfold phases are those of the generic pass (chz_pass.h,
 // chz_device.h).  What differs is how a tile moves between HBM and LDS:
-//   * the tile is point-major with the lines of each point row permuted by an
-//     XOR swizzle (STile): the FFT phases (lanes = lines) still read whole
-//     point rows, and the row staging of the x passes (consecutive lanes =
+//   * in the x passes the tile is point-major with the lines of each point
+//     row permuted by an XOR swizzle (STile): the FFT phases (lanes = lines)
+//     still read whole point rows, and the row staging (consecutive lanes =
 //     consecutive slots of one line) spreads over all LDS banks instead of
 //     hitting a few;
 //   * every global access of a tile is a wave-uniform 64-bit base plus a
@@ -20,40 +20,45 @@
 namespace chz {
 namespace line {
 
-// Line permutation of point row k: L = 8 swaps by (k >> 1) & 7, so the 16
-// consecutive points of a row-staging instruction land in 16 distinct 16-byte
-// bank groups; L = 16 rows are 256 bytes and permute by k & 15.
-template <int L> __device__ __forceinline__ constexpr int swz(int k)
+// Line permutation of point row k for the row passes: L = 8 swaps by
+// (k >> 1) & 7, so the 16 consecutive points of a row-staging instruction land
+// in 16 distinct 16-byte bank groups; L = 16 rows are 256 bytes and permute by
+// k & 15.  Column passes stage with lanes = lines, which is conflict-free
+// unpermuted, and keep the plain layout (no index arithmetic in the FFT).
+template <int L, bool SW> __device__ __forceinline__ constexpr int swz(int k)
 {
-    if constexpr (L == 8) return (k >> 1) & 7;
+    if constexpr (!SW) return 0;
+    else if constexpr (L == 8) return (k >> 1) & 7;
     else return k & (L - 1);
 }
 
-template <typename T, int LINES> struct STile {
+template <typename T, int LINES, bool SW> struct STile {
     typedef typename vec2_of<T>::type V;
     static constexpr int L = LINES;
     V *buf;
     int c;
-    __device__ __forceinline__ int idx(int k) const { return k * L + (c ^ swz<L>(k)); }
+    __device__ __forceinline__ int idx(int k) const { return k * L + (c ^ swz<L, SW>(k)); }
     __device__ __forceinline__ V &pt(int k) const { return buf[idx(k)]; }
     __device__ __forceinline__ T &re(int s) const { return reinterpret_cast<T *>(buf)[2 * idx(s >> 1) + (s & 1)]; }
 };
 
-template <typename T, int N, int TPL, int LINES> struct Io {
+template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     typedef typename vec2_of<T>::type V;
-    typedef STile<T, LINES> TL;
+    typedef STile<T, LINES, SW> TL;
     static constexpr int L = LINES, M = N / 2, NS = N + 2;
     static constexpr int NCH = (NS + TPL - 1) / TPL;   // slot chunks of TPL per line
-    static_assert(TPL % 32 == 0, "threads per line: a multiple of 32 (chunk steps keep the row swizzle)");
+    static_assert(!SW || TPL % 32 == 0, "swizzled tiles: threads per line a multiple of 32 (chunk steps keep the row swizzle)");
+    static_assert(TPL % 2 == 0, "threads per line: even (a chunk step keeps the slot parity)");
     // chunk i (slots i*TPL .. i*TPL + TPL - 1) lies inside every window
     static constexpr bool interior(int i) { return i * TPL >= 2 && i * TPL + TPL - 1 <= N - 2; }
 
     // LDS element of slot s0 + i*TPL of line c is the per-lane base plus
-    // i * lds_step: s0 < TPL and TPL is a multiple of 32, so a chunk step moves
-    // the point index by a multiple of 16 and leaves the row swizzle unchanged
+    // i * lds_step: s0 < TPL, TPL is even (a multiple of 32 when swizzled), so
+    // a chunk step keeps the slot parity and moves the point index by TPL/2
+    // (a multiple of 16), which leaves the row swizzle unchanged
     __device__ __forceinline__ static T *lds_base(V *buf, int c, int s0)
     {
-        return reinterpret_cast<T *>(buf) + 2 * ((s0 >> 1) * L + (c ^ swz<L>(s0 >> 1))) + (s0 & 1);
+        return reinterpret_cast<T *>(buf) + 2 * ((s0 >> 1) * L + (c ^ swz<L, SW>(s0 >> 1))) + (s0 & 1);
     }
     static constexpr int lds_step = TPL * L;   // doubles per chunk
 
@@ -149,12 +154,14 @@ template <typename T, int N, int TPL, int LINES> struct Io {
 
     // ---- store ---------------------------------------------------------------
     // slots [st_lo, st_hi] of every line, value * scale inside [out_lo, out_hi], 0 outside
+    // negodd: the C2R unfold (odd slots negated: the real view of conj(FFT)),
+    // applied here instead of in LDS
     template <bool FULL>
-    __device__ __forceinline__ static void store_rows(const PassArgs &p, V *buf, int nlines, T *gbase, int t)
+    __device__ __forceinline__ static void store_rows(const PassArgs &p, V *buf, int nlines, T *gbase, int t, bool negodd)
     {
         const int r = t / TPL, l = t % TPL;
         if (!FULL && r >= nlines) return;
-        const T sc = (T)p.scale;
+        const T sc = (negodd && (l & 1)) ? -(T)p.scale : (T)p.scale;   // slot parity = l parity (TPL even)
         const int hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
         T *rowp = gbase + (unsigned)(r * (int)p.d_sa + l);
         const T *lp = lds_base(buf, r, l);
@@ -168,11 +175,11 @@ template <typename T, int N, int TPL, int LINES> struct Io {
         }
     }
     template <bool FULL>
-    __device__ __forceinline__ static void store_cols(const PassArgs &p, V *buf, int nlines, T *gbase, int t)
+    __device__ __forceinline__ static void store_cols(const PassArgs &p, V *buf, int nlines, T *gbase, int t, bool negodd)
     {
         const int c = t % L, u = t / L;
         if (!FULL && c >= nlines) return;
-        const T sc = (T)p.scale;
+        const T sc = (negodd && (u & 1)) ? -(T)p.scale : (T)p.scale;   // slot parity = u parity (TPL even)
         const unsigned lane = (unsigned)(c * (int)p.d_sa) + (unsigned)u * (unsigned)p.d_sp;
         const long long step = (long long)TPL * p.d_sp;
         const T *lp = lds_base(buf, c, u);
@@ -212,7 +219,7 @@ __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, 
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int a_tile, int b, int nlines, unsigned char *smem)
 {
-    typedef Io<T, N, TPL, L> IO;
+    typedef Io<T, N, TPL, L, OP == B_ROWS || OP == B_XFWD> IO;
     typedef typename IO::V V;
     typedef typename IO::TL TL;
     constexpr int M = N / 2;
@@ -230,35 +237,59 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int 
     if constexpr (OP == B_XFWD) IO::stage_xfwd(sa, buf, nlines, a_tile, b, t);
     __syncthreads();
     PassArgs q = p;
+    // the last transform of the pass; a C2R one leaves its unfold (a sign
+    // flip of the odd slots) to the store
+    int last = p.kind;
     if constexpr (OP == B_ZMID) {
+        last = sa.kind2;
         if (p.kind == K_R2C && sa.kind2 == K_C2R) {
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
             periodic_split_divide<EPT, T>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
-            unfold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
         } else {
             line_transform<EPT, T>(p.kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
             zdivide<T>(p, sa, tl, u, TPL, nlines, a_tile, b);
             __syncthreads();
-            line_transform<EPT, T>(sa.kind2, tl, u, TPL, M, N, PLAN, tw, twl, red);
+            if (last == K_C2R) {
+                fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
+                fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+            } else {
+                line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tw, twl, red);
+            }
         }
         q.out_lo = sa.out2_lo;
         q.out_hi = sa.out2_hi;
     } else {
-        line_transform<EPT, T>(p.kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
+        if (last == K_C2R) {
+            fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
+            fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+        } else {
+            line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tw, twl, red);
+        }
     }
     T *dst = reinterpret_cast<T *>(p.dst) + (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
-    if constexpr (OP == B_ROWS || OP == B_XFWD) IO::template store_rows<FULL>(q, buf, nlines, dst, t);
-    else IO::template store_cols<FULL>(q, buf, nlines, dst, t);
+    if constexpr (OP == B_ROWS || OP == B_XFWD) IO::template store_rows<FULL>(q, buf, nlines, dst, t, last == K_C2R);
+    else IO::template store_cols<FULL>(q, buf, nlines, dst, t, last == K_C2R);
 }
 
 }  // namespace line
+
+// Waves per SIMD that the LDS of one tile leaves room for (160 KB per CU, at
+// most 4): the register budget of the kernel follows from it.
+template <typename T, int N, int TPL, int L> constexpr int line_occupancy()
+{
+    constexpr long lds = ((long)(N / 2 + 1) * L + N / 2) * 2 * (long)sizeof(T) + (long)TPL * L * (long)sizeof(T);
+    constexpr long wgs = 163840 / lds;
+    constexpr long w = wgs * (TPL * L / 64) / 4;
+    return w < 1 ? 1 : w > 4 ? 4 : (int)w;
+}
 
 // Fixed-shape pass kernel with lean tile I/O (same arguments as k_fixed).
 // Precondition (checked by the host): every window of the pass contains
 // slots 2..N-2, which holds for every kind and solve pass.
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
-__global__ void __launch_bounds__(L *TPL, 4) k_line(PassArgs p, SolveArgs sa, const double *, double *)
+__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L>())) k_line(PassArgs p, SolveArgs sa,
+                                                                                  const double *, double *)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     PassArgs q = p;
