@@ -171,6 +171,43 @@ def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
     return f
 
 
+def hw3crt(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3, ldimf=None, mdimf=None):
+    """hw3crt_ (include/hw3crt_wrapper.h; the reference's cudahelmholtz/hw3crt_wrapper.h:1-9)
+    on host numpy arrays, solved on the GPU.  f: float64 array F(LDIMF, MDIMF, N+1) in
+    FISHPACK order, i.e. numpy shape (NZ+1, MDIMF, LDIMF) (C order), updated in place.
+    bcs = (bdxs, bdxf, bdys, bdyf, bdzs, bdzf) face arrays in FISHPACK's layouts
+    ((N+1, MDIMF), (N+1, LDIMF), (M+1, LDIMF)) or None.  Returns (pertrb, ierror)."""
+    import numpy as np
+    L = lib()
+    NX, NY, NZ = (int(v) for v in n)
+    ld = NX + 1 if ldimf is None else int(ldimf)
+    md = NY + 1 if mdimf is None else int(mdimf)
+    if f.dtype != np.float64 or not f.flags["C_CONTIGUOUS"] or f.size < ld * md * (NZ + 1):
+        raise ValueError("f must be a C-contiguous float64 array of LDIMF*MDIMF*(NZ+1) values")
+    dp = ctypes.POINTER(ctypes.c_double)
+    keep = []
+
+    def arr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(dp)
+    D, I = ctypes.c_double, ctypes.c_int
+    xs, xf, ys, yf, zs, zf = (D(float(v)) for v in dom)
+    l, m, nn = I(NX), I(NY), I(NZ)
+    lb, mb, nb = (I(int(b)) for b in bc)
+    el, ldi, mdi = D(float(lam)), I(ld), I(md)
+    pert, ierr = D(0.0), I(-1)
+    fb = [arr(b) for b in bcs]
+    R = ctypes.byref
+    L.hw3crt_(R(xs), R(xf), R(l), R(lb), fb[0], fb[1], R(ys), R(yf), R(m), R(mb), fb[2], fb[3],
+              R(zs), R(zf), R(nn), R(nb), fb[4], fb[5], R(el), R(ldi), R(mdi), f.ctypes.data_as(dp),
+              R(pert), R(ierr), None)
+    del keep
+    return pert.value, ierr.value
+
+
 def helmholtz_adjoint(u, gu, n, bc, lam, dom=(0.0, 1.0) * 3, faces=False, want_lambda=True):
     """Vector-Jacobian product of helmholtz_solve (chz_helmholtz_adjoint).
 

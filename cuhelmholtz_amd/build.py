@@ -1,7 +1,8 @@
 """Build libcuhelmholtz_amd.so in-tree with hipcc for gfx950.
 
 Each source under csrc/ is compiled to an object in csrc/../_build (in
-parallel), then linked into cuhelmholtz_amd/libcuhelmholtz_amd.so.
+parallel), then linked into cuhelmholtz_amd/libcuhelmholtz_amd.so; the
+command-line drivers (drivers/*.cpp) are linked against it into bin/.
 Usage: python -m cuhelmholtz_amd.build  (or build() from Python).
 """
 import concurrent.futures as cf
@@ -14,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuhelmholtz_amd.so")
+DRIVERS = os.path.join(HERE, "drivers")
+BIN = os.path.join(HERE, "bin")
 ARCH = os.environ.get("CHZ_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
@@ -60,9 +63,23 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_drivers()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources, drivers in {BIN}")
     return LIB
+
+
+def build_drivers():
+    """The command-line drivers (drivers/*.cpp, the reference's test programs'
+    arguments and output) linked against the library, into bin/."""
+    os.makedirs(BIN, exist_ok=True)
+    for src in sorted(f for f in os.listdir(DRIVERS) if f.endswith(".cpp")):
+        out = os.path.join(BIN, src[:-4])
+        cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), "-I", DRIVERS,
+               os.path.join(DRIVERS, src), "-L", HERE, "-lcuhelmholtz_amd", "-Wl,-rpath,$ORIGIN/..", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -26,7 +26,9 @@ def test_cpp_dropin_symbols_exported():
                 "_Z20do_transform_nocubicPdS_iii14transform_kindS0_S0_",
                 "_Z18do_transform_cubicPdiii14transform_kindS0_S0_",
                 "_Z16freeMemory_cubicv",
-                "_Z21cuda_helmholtz_solverddiiPdS_ddiiS_S_ddiiS_S_dS_"]:
+                "_Z21cuda_helmholtz_solverddiiPdS_ddiiS_S_ddiiS_S_dS_",
+                # hw3crt_wrapper.h: FISHPACK's hw3crt_ (C linkage) and the C++ solver_3d wrapper
+                "hw3crt_", "_Z9solver_3dddiiPdS_ddiiS_S_ddiiS_S_dS_"]:
         assert hasattr(L, sym), sym
 
 
