@@ -60,6 +60,8 @@ def _declare_solver(L):
     vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
     if hasattr(L, "chz_helmholtz_solve"):
         L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
+    if hasattr(L, "chz_helmholtz_solve19"):
+        L.chz_helmholtz_solve19.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve_mixed"):
         L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
     if hasattr(L, "chz_helmholtz_adjoint"):
@@ -152,10 +154,11 @@ def _faces(bcs, n, bc, dev):
     return out, keep
 
 
-def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
+def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3, stencil=7):
     """In-place FFT-diagonalised solve of laplace u + lam u = f on a CUDA float64
     node tensor f shaped (NZ+1, NY+1, NX+1) (cuda_helmholtz_solver semantics).
-    bcs = (bcl, bcr, bcb, bct, bcf, bce) Neumann face arrays or None."""
+    bcs = (bcl, bcr, bcb, bct, bcf, bce) Neumann face arrays or None.
+    stencil=19: the compact 19-point operator (chz_helmholtz_solve19)."""
     import torch
     NX, NY, NZ = (int(v) for v in n)
     if not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous():
@@ -166,7 +169,10 @@ def helmholtz_solve(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
     args = []
     for a, N in enumerate((NX, NY, NZ)):
         args += [float(dom[2 * a]), float(dom[2 * a + 1]), N, int(bc[a]), faces[2 * a], faces[2 * a + 1]]
-    _check(lib().chz_helmholtz_solve(*args, float(lam), ctypes.c_void_p(f.data_ptr()), _stream(f)))
+    if stencil not in (7, 19):
+        raise ValueError("stencil must be 7 or 19")
+    fn = lib().chz_helmholtz_solve if stencil == 7 else lib().chz_helmholtz_solve19
+    _check(fn(*args, float(lam), ctypes.c_void_p(f.data_ptr()), _stream(f)))
     del keep
     return f
 

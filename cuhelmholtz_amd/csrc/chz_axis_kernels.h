@@ -177,8 +177,44 @@ __device__ __forceinline__ double recip(double d)
     return r;
 }
 
+// Spectral divide factor of one z line (x, y): 1 / (e_x + e_y + e_z + lambda)
+// for the 7-point operator; for the 19-point operator (the reference's
+// _nonpoisson_devide_19points_, cudahelmholtz/_cuda_helmholtz_.inl:255-274,
+// generalised to every boundary code through the axis eigenvalues e_a and to
+// unequal spacings) B / (L19 + lambda) with
+//   L19 = e_x + e_y + e_z + c_xy e_x e_y + c_xz e_x e_z + c_yz e_y e_z,
+//   B   = 1 + b_x e_x + b_y e_y + b_z e_z,
+// c_ab = (h_a^2 + h_b^2) / 12, b_a = h_a^2 / 12 (for h_a = h and
+// e_a = (2 cos t_a - 2) / h^2 these are the reference's 2/3 (sum cos cos +
+// sum cos) - 4 over h^2 and 1/2 + 1/6 sum cos).
+// S19 is a compile-time choice (the callers branch once per tile), so the
+// unrolled divide loops carry one reciprocal per value.
+template <bool S19> struct LineDivide {
+    double a, cz, b0, bz, lam;
+    __device__ __forceinline__ LineDivide(const SolveArgs &sa, double ex, double ey)
+    {
+        lam = sa.lambda;
+        if constexpr (!S19) {
+            a = ex + ey;
+            cz = 1.0;
+            b0 = 1.0;
+            bz = 0.0;
+        } else {
+            a = ex + ey + sa.c19[0] * ex * ey;
+            cz = 1.0 + sa.c19[1] * ex + sa.c19[2] * ey;
+            b0 = 1.0 + sa.c19[3] * ex + sa.c19[4] * ey;
+            bz = sa.c19[5];
+        }
+    }
+    __device__ __forceinline__ double operator()(double ez) const
+    {
+        if constexpr (!S19) return recip(a + ez + lam);
+        else return (b0 + bz * ez) * recip(a + cz * ez + lam);
+    }
+};
+
 // the divide of the fused z pass on the tile of z lines (x = a_tile + c, y)
-template <typename T>
+template <typename T, bool S19 = false>
 __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &sa, const Tile<T> &tl, int u, int nlines,
                                             int a_tile, int y)
 {
@@ -186,14 +222,14 @@ __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &
     {
         const int x = a_tile + c;
         const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
-        const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
+        const LineDivide<S19> dv(sa, c < nlines ? ex[x] : 0.0, c < nlines ? ey[y] : 0.0);
         const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
         T *rbuf = reinterpret_cast<T *>(tl.buf);
 #pragma unroll 4
         for (int s = p.out_lo + u; s <= p.out_hi; s += p.TPL) {
             T &v = rbuf[(size_t)((s >> 1) * tl.LS + c) * 2 + (s & 1)];
             const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
-            const double r = m0 ? 0.0 : recip(exy + ez[s] + sa.lambda);
+            const double r = m0 ? 0.0 : dv(ez[s]);
             v = (T)((double)v * r);
         }
     }
@@ -205,7 +241,7 @@ __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &
 // the point pairs (k, M - k), so they run as one pass: read Z_k, Z_{M-k},
 // form X_k, X_{M-k}, scale, recombine into the (conjugated) inverse-FFT input.
 // Same arithmetic as unfold<K_R2C> -> zmid_divide -> fold<K_C2R>.
-template <int EPT, typename T, typename TL>
+template <int EPT, typename T, typename TL, bool S19 = false>
 __device__ __forceinline__ void periodic_split_divide(const TL &tl, int u, int TPL, int M, int nlines, int a_tile,
                                                       int y, const SolveArgs &sa,
                                                       const typename vec2_of<T>::type *__restrict__ tw)
@@ -215,7 +251,7 @@ __device__ __forceinline__ void periodic_split_divide(const TL &tl, int u, int T
     const T h = (T)0.5;
     const int c = tl.c, x = a_tile + c;
     const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
-    const double exy = c < nlines ? ex[x] + ey[y] + sa.lambda : 1.0;
+    const LineDivide<S19> dv(sa, c < nlines ? ex[x] : 0.0, c < nlines ? ey[y] : 0.0);
     const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
     V ca[PP], cb[PP], wk[PP];
 #pragma unroll
@@ -235,8 +271,8 @@ __device__ __forceinline__ void periodic_split_divide(const TL &tl, int u, int T
             const V P = mk<V>(w.x * O.x + w.y * O.y, w.x * O.y - w.y * O.x);
             V Xk = mk<V>(E.x + P.x, E.y + P.y), Xm = mk<V>(E.x - P.x, -(E.y - P.y));
             // divide by the eigenvalue sums of modes k and M - k (mode m = slots 2m, 2m + 1)
-            const double rk = (m0xy && k == 0) ? 0.0 : recip(exy + ez[2 * k]);
-            const double rm = recip(exy + ez[2 * (M - k)]);
+            const double rk = (m0xy && k == 0) ? 0.0 : dv(ez[2 * k]);
+            const double rm = dv(ez[2 * (M - k)]);
             Xk = mk<V>((T)((double)Xk.x * rk), (T)((double)Xk.y * rk));
             Xm = mk<V>((T)((double)Xm.x * rm), (T)((double)Xm.y * rm));
             if (k == 0) {   // DC and Nyquist are real
@@ -285,12 +321,14 @@ __device__ __forceinline__ void body_zmid(const PassArgs &p, const SolveArgs &sa
     __syncthreads();
     if (p.kind == K_R2C && sa.kind2 == K_C2R) {
         fft_forward<EPT, T>(tl, u, p.TPL, p.M, p.plan, sm.twl);
-        periodic_split_divide<EPT, T>(tl, u, p.TPL, p.M, nlines, a_tile, y, sa, tw);
+        if (sa.stencil19) periodic_split_divide<EPT, T, Tile<T>, true>(tl, u, p.TPL, p.M, nlines, a_tile, y, sa, tw);
+        else periodic_split_divide<EPT, T>(tl, u, p.TPL, p.M, nlines, a_tile, y, sa, tw);
         fft_forward<EPT, T>(tl, u, p.TPL, p.M, p.plan, sm.twl);
         unfold<K_C2R, EPT, T>(tl, u, p.TPL, p.M, p.N, tw, sm.red);
     } else {
         line_transform<EPT, T>(p.kind, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
-        zmid_divide<T>(p, sa, tl, u, nlines, a_tile, y);
+        if (sa.stencil19) zmid_divide<T, true>(p, sa, tl, u, nlines, a_tile, y);
+        else zmid_divide<T>(p, sa, tl, u, nlines, a_tile, y);
         __syncthreads();
         line_transform<EPT, T>(sa.kind2, tl, u, p.TPL, p.M, p.N, p.plan, tw, sm.twl, sm.red);
     }

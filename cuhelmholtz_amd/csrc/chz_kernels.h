@@ -58,6 +58,10 @@ struct SolveArgs {
     int kind2, out2_lo, out2_hi;                       // inverse kind of the middle pass and its window
     const double *pertrb;                              // singular case: weighted sum of the folded rhs (or null)
     double pert_scale;                                 // 1 / (sum of the 0-mode weights)
+    // 19-point operator (0: the 7-point one): divide by (L19 + lambda) / B with
+    // L19 = sum e_a + sum_{a<b} c_ab e_a e_b, B = 1 + sum_a b_a e_a; c19 = {c_xy, c_xz, c_yz, b_x, b_y, b_z}
+    int stencil19;
+    double c19[6];
 };
 
 // Point stride of a tile of L lines in LDS: FP64 tiles are unpadded (a

@@ -196,20 +196,20 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
 };
 
 // the divide of the fused z pass (zmid_divide of the generic pass) on the swizzled tile
-template <typename T, typename TLT>
+template <typename T, bool S19, typename TLT>
 __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, const TLT &tl, int u, int TPL, int nlines,
                                         int a_tile, int y)
 {
     const int c = tl.c;
     const int x = a_tile + c;
     const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
-    const double exy = c < nlines ? ex[x] + ey[y] : 0.0;
+    const LineDivide<S19> dv(sa, c < nlines ? ex[x] : 0.0, c < nlines ? ey[y] : 0.0);
     const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
 #pragma unroll 4
     for (int s = p.out_lo + u; s <= p.out_hi; s += TPL) {
         T &v = tl.re(s);
         const bool m0 = m0xy && (sa.bc[2] == 0 ? s < 2 : s == 0);
-        const double r = m0 ? 0.0 : recip(exy + ez[s] + sa.lambda);
+        const double r = m0 ? 0.0 : dv(ez[s]);
         v = (T)((double)v * r);
     }
 }
@@ -244,11 +244,13 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int 
         last = sa.kind2;
         if (p.kind == K_R2C && sa.kind2 == K_C2R) {
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
-            periodic_split_divide<EPT, T>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
+            if (sa.stencil19) periodic_split_divide<EPT, T, TL, true>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
+            else periodic_split_divide<EPT, T, TL, false>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
         } else {
             line_transform<EPT, T>(p.kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
-            zdivide<T>(p, sa, tl, u, TPL, nlines, a_tile, b);
+            if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b);
+            else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b);
             __syncthreads();
             if (last == K_C2R) {
                 fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);

@@ -66,6 +66,7 @@ int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3]
         g.faces[2 * a] = need_lo ? faces[2 * a] : nullptr;
         g.faces[2 * a + 1] = need_hi ? faces[2 * a + 1] : nullptr;
     }
+    g.stencil = 7;
     g.lambda = lambda;
     g.singular = lambda == 0.0 && (bc[0] == 0 || bc[0] == 3) && (bc[1] == 0 || bc[1] == 3) && (bc[2] == 0 || bc[2] == 3);
     // ratio = 1 / prod(N_a/2 * (2 if periodic)), cuda_helmholtz_solver.cu:211-215
@@ -121,6 +122,17 @@ void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const d
     sa.eig = eig;
     sa.lambda = g.lambda;
     sa.singular = g.singular ? 1 : 0;
+    if (g.stencil == 19) {
+        // L19 = sum e_a + sum_{a<b} (h_a^2 + h_b^2)/12 e_a e_b, B = 1 + sum h_a^2/12 e_a
+        const double hx = 1.0 / g.ih2[0], hy = 1.0 / g.ih2[1], hz = 1.0 / g.ih2[2];
+        sa.stencil19 = 1;
+        sa.c19[0] = (hx + hy) / 12.0;
+        sa.c19[1] = (hx + hz) / 12.0;
+        sa.c19[2] = (hy + hz) / 12.0;
+        sa.c19[3] = hx / 12.0;
+        sa.c19[4] = hy / 12.0;
+        sa.c19[5] = hz / 12.0;
+    }
 }
 
 static int check_spec(const PassSpec &q, long long svol, long long dvol, const char *what)
@@ -345,12 +357,14 @@ int periodic_copies(const SolveGeom &g, double *f, hipStream_t st)
 }
 
 int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
-                   double *f, hipStream_t st)
+                   double *f, hipStream_t st, int stencil)
 {
     if (!f) return fail(ERR_INVALID, "null solution array");
+    if (stencil != 7 && stencil != 19) return fail(ERR_INVALID, "stencil must be 7 or 19");
     SolveGeom g;
     int rc = make_geom(g, dom, N, bc, lambda, faces);
     if (rc) return rc;
+    g.stencil = stencil;
     Context &ctx = Context::get();
     std::lock_guard<std::mutex> lock(ctx.mutex());
     const size_t bytes = sizeof(double) * (size_t)ws_row(N[0], sizeof(double)) * (N[1] + 2) * (N[2] + 2);
@@ -440,6 +454,17 @@ int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double
     const int N[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
     const double *faces[6] = {bcl, bcr, bcb, bct, bcf, bce};
     return chz::helmholtz_fp64(dom, N, bc, faces, lambda, f, (hipStream_t)stream);
+}
+
+int chz_helmholtz_solve19(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                          double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                          double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                          double lambda, double *f, void *stream)
+{
+    const double dom[6] = {left, right, bottom, top, front, back};
+    const int N[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
+    const double *faces[6] = {bcl, bcr, bcb, bct, bcf, bce};
+    return chz::helmholtz_fp64(dom, N, bc, faces, lambda, f, (hipStream_t)stream, 19);
 }
 
 }  // extern "C"

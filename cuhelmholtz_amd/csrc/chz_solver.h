@@ -16,6 +16,7 @@ struct SolveGeom {
     bool singular;
     double ratio;
     const double *faces[6];   // bcl, bcr, bcb, bct, bcf, bce (null where unused)
+    int stencil;              // 7 (the reference's operator) or 19 (its unlaunched compact variant)
 };
 
 long long ws_row(int NX, size_t esz);
@@ -47,6 +48,6 @@ int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
                     double *f, int refine, hipStream_t st);
 int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
-                   double *f, hipStream_t st);
+                   double *f, hipStream_t st, int stencil = 7);
 
 }  // namespace chz

@@ -57,6 +57,22 @@ int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double
                         double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
                         double lambda, double *f, void *stream);
 
+/* The same solve with the 19-point compact operator: (L19 + lambda) u = B f,
+ * L19 = sum_a D_a + sum_{a<b} (h_a^2 + h_b^2)/12 D_a D_b and B = I + sum_a
+ * h_a^2/12 D_a, D_a the second difference along axis a with the boundary
+ * closure of its code; the boundary data are folded into f as for the
+ * 7-point operator.  Replaces the reference's _nonpoisson_devide_19points_
+ * (cudahelmholtz/_cuda_helmholtz_.inl:255-274), which is defined but never
+ * launched (cuda_helmholtz_solver.cu:325,338) and hard-codes cos(i pi/N) for
+ * every axis (codes 1 and 3 only); here every code uses its own eigenvalues.
+ * Fourth order for lambda = 0 on problems whose boundary extension is smooth
+ * (periodic, homogeneous data); second order otherwise (lambda u is not
+ * compact-corrected, as in the reference's kernel). */
+int chz_helmholtz_solve19(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                          double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                          double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                          double lambda, double *f, void *stream);
+
 /* Same solve with FP32 transforms and divide plus `refine` (0..16) steps of
  * FP64 iterative refinement: an FP32 solve of the folded rhs, then per step
  * the FP64 7-point residual r = f - (Delta_h u + lambda u), an FP32

@@ -42,6 +42,7 @@ def lib():
         L.oracle_transform.argtypes = [dp] + [ctypes.c_int] * 6
         L.oracle_helmholtz.argtypes = ([ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp] * 3
                                        + [ctypes.c_double, dp])
+        L.oracle_helmholtz19.argtypes = L.oracle_helmholtz.argtypes
         L.oracle_bc_fold.argtypes = ([ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp] * 3
                                      + [dp])
         L.solver_3d.argtypes = ([ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp] * 3
@@ -88,12 +89,14 @@ def _bcargs(dom, n, bc, bcs):
     return out
 
 
-def helmholtz(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
+def helmholtz(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3, stencil=7):
     """Restated cuda_helmholtz_solver on a node array f shaped (NZ+1, NY+1, NX+1).
-    bcs = (bcl, bcr, bcb, bct, bcf, bce) face arrays or None. Returns u."""
+    bcs = (bcl, bcr, bcb, bct, bcf, bce) face arrays or None. Returns u.
+    stencil=19: the 19-point compact operator (oracle_helmholtz19)."""
     u = np.ascontiguousarray(f, dtype=np.float64).copy()
     keep = [None if b is None else np.ascontiguousarray(b, dtype=np.float64) for b in bcs]
-    if lib().oracle_helmholtz(*_bcargs(dom, n, bc, keep), float(lam), _p(u)) != 0:
+    fn = lib().oracle_helmholtz if stencil == 7 else lib().oracle_helmholtz19
+    if fn(*_bcargs(dom, n, bc, keep), float(lam), _p(u)) != 0:
         raise ValueError("bad solver arguments")
     return u
 

@@ -21,6 +21,9 @@
  *                       residual invariant, O(h^2) manufactured solutions
  *   hw3crt_/solver_3d -- same as oracle_helmholtz (different algorithm: x/y
  *                       transforms + z tridiagonal), cross-checked at 1e-12
+ *   oracle_helmholtz19 -- the explicit 19-point stencil and rhs operator applied
+ *                       in physical space (periodic, homogeneous Dirichlet and
+ *                       Neumann closures), O(h^4) at lambda = 0
  */
 #include <math.h>
 #include <stdio.h>
@@ -417,10 +420,10 @@ static void eig_table(int bc, int N, double ih2, double *t)
 /* cuda_helmholtz_solver restated on host arrays (same argument list,
  * cudahelmholtz/cuda_helmholtz_solver.h:5-11). f is (NX+1)(NY+1)(NZ+1) nodes,
  * x fastest; the solution overwrites it.  Returns 0, or -1 on bad input. */
-int oracle_helmholtz(double left, double right, int NX, int xbc, double *bcl, double *bcr,
-                     double bottom, double top, int NY, int ybc, double *bcb, double *bct,
-                     double front, double back, int NZ, int zbc, double *bcf, double *bce,
-                     double lambda, double *f)
+static int helmholtz_impl(double left, double right, int NX, int xbc, double *bcl, double *bcr,
+                          double bottom, double top, int NY, int ybc, double *bcb, double *bct,
+                          double front, double back, int NZ, int zbc, double *bcf, double *bce,
+                          double lambda, double *f, int stencil)
 {
     int Nn[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
     int s[3], e[3], fwd[3], inv[3];
@@ -462,6 +465,17 @@ int oracle_helmholtz(double left, double right, int NX, int xbc, double *bcl, do
                 int m0 = xbc == 0 ? i / 2 : i, m1 = ybc == 0 ? j / 2 : j, m2 = zbc == 0 ? k / 2 : k;
                 double eig;
                 if (needperturb && m0 == 0 && m1 == 0 && m2 == 0) eig = 0.;
+                else if (stencil == 19) {
+                    /* _nonpoisson_devide_19points_ (.inl:255-274): divide by the
+                     * 19-point symbol + lambda, multiply by the rhs operator's
+                     * symbol; written with the axis eigenvalues e_a = tx, ty, tz
+                     * (h_a^2 e_a = 2 cos - 2) instead of cos(i pi / N) */
+                    double h2[3] = {1. / (ih[0] * ih[0]), 1. / (ih[1] * ih[1]), 1. / (ih[2] * ih[2])};
+                    double L19 = tx[i] + ty[j] + tz[k] + (h2[0] + h2[1]) / 12. * tx[i] * ty[j] +
+                                 (h2[0] + h2[2]) / 12. * tx[i] * tz[k] + (h2[1] + h2[2]) / 12. * ty[j] * tz[k];
+                    double B = 1. + (h2[0] * tx[i] + h2[1] * ty[j] + h2[2] * tz[k]) / 12.;
+                    eig = B / (L19 + lambda);
+                }
                 else eig = 1. / (tx[i] + ty[j] + tz[k] + lambda);
                 ex[i + px * (j + py * k)] *= eig;
             }
@@ -488,6 +502,26 @@ int oracle_helmholtz(double left, double right, int NX, int xbc, double *bcl, do
         for (int j = 0; j <= NY; j++) for (int i = 0; i <= NX; i++)
             f[i + fx * (j + fy * NZ)] = f[i + fx * (j + fy * 0)];
     return 0;
+}
+
+int oracle_helmholtz(double left, double right, int NX, int xbc, double *bcl, double *bcr,
+                     double bottom, double top, int NY, int ybc, double *bcb, double *bct,
+                     double front, double back, int NZ, int zbc, double *bcf, double *bce,
+                     double lambda, double *f)
+{
+    return helmholtz_impl(left, right, NX, xbc, bcl, bcr, bottom, top, NY, ybc, bcb, bct, front, back, NZ, zbc,
+                          bcf, bce, lambda, f, 7);
+}
+
+/* the same solve with the 19-point compact operator of the reference's
+ * (unlaunched) _nonpoisson_devide_19points_, cudahelmholtz/_cuda_helmholtz_.inl:255-274 */
+int oracle_helmholtz19(double left, double right, int NX, int xbc, double *bcl, double *bcr,
+                       double bottom, double top, int NY, int ybc, double *bcb, double *bct,
+                       double front, double back, int NZ, int zbc, double *bcf, double *bce,
+                       double lambda, double *f)
+{
+    return helmholtz_impl(left, right, NX, xbc, bcl, bcr, bottom, top, NY, ybc, bcb, bct, front, back, NZ, zbc,
+                          bcf, bce, lambda, f, 19);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -73,6 +73,22 @@ def test_all_bc_vs_oracle(n, dom):
         assert rel(got, want) < TOL, bc
 
 
+@pytest.mark.parametrize("n,dom", [((6, 8, 10), (0.0, 1.0) * 3), ((12, 4, 6), (0.0, 2.0, -1.0, 0.5, 0.25, 1.0))])
+def test_19point_all_bc_vs_oracle(n, dom):
+    """The 19-point operator (chz_helmholtz_solve19) for every code triple, with
+    boundary data folded, against the oracle (pinned by the physical-space stencil)."""
+    rng = np.random.default_rng(19)
+    for bc in itertools.product(range(5), repeat=3):
+        for lam in (-2.3, 0.0):
+            f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+            bcs = faces(rng, n, bc)
+            ft = torch.from_numpy(f.copy()).to(dev())
+            C.helmholtz_solve(ft, n, bc, lam, bcs, dom, stencil=19)
+            torch.cuda.synchronize()
+            want = O.helmholtz(f, n, bc, lam, bcs, dom, stencil=19)
+            assert rel(ft.cpu().numpy(), want) < TOL, (bc, lam)
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_long_axis_every_bc_vs_oracle(axis):
     """512 cells along one axis (the fixed-shape pass kernels of that axis's
@@ -92,6 +108,11 @@ def test_long_axis_every_bc_vs_oracle(axis):
             got = gpu_solve(f, n, bc, lam, bcs)
             want = O.helmholtz(f, n, bc, lam, bcs)
             assert rel(got, want) < TOL, (n, bc, lam)
+            # the 19-point operator through the same pass kernels
+            ft = torch.from_numpy(f.copy()).to(dev())
+            C.helmholtz_solve(ft, n, bc, lam, bcs, stencil=19)
+            torch.cuda.synchronize()
+            assert rel(ft.cpu().numpy(), O.helmholtz(f, n, bc, lam, bcs, stencil=19)) < TOL, (n, bc, lam, 19)
 
 
 @pytest.mark.parametrize("bc", [(0, 0, 0), (3, 3, 3), (0, 3, 0), (3, 0, 3)])

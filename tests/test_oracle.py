@@ -256,3 +256,73 @@ def test_checks_catch_planted_mistakes():
     k = [int(v) for v in g["kinds_1"]]
     y_sw = O.transform(g["x_1"], k[1], k[0], k[2])
     assert rel(y_sw[1:6, 1:8, 1:12], g["y_1"][1:6, 1:8, 1:12]) > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# 19-point compact operator (the reference's unlaunched
+# _nonpoisson_devide_19points_, cudahelmholtz/_cuda_helmholtz_.inl:255-274)
+# ---------------------------------------------------------------------------
+def _closure_d2(v, axis, code, h):
+    """Second difference along `axis` of a grid function on the unknown box, with
+    the code's homogeneous closure: periodic wrap (0), zero beyond Dirichlet ends
+    (1, 2 low, 4 high), mirror ghost u_-1 = u_1 at Neumann ends (3, 2 high, 4 low)."""
+    v = np.moveaxis(v, axis, 0)
+    n = v.shape[0]
+    if code == 0:
+        lo, hi = v[-1:], v[:1]
+    else:
+        neu_lo, neu_hi = code in (3, 4), code in (2, 3)
+        lo = v[1:2] if neu_lo else np.zeros_like(v[:1])
+        hi = v[n - 2:n - 1] if neu_hi else np.zeros_like(v[:1])
+    ext = np.concatenate([lo, v, hi], axis=0)
+    d = (ext[2:] - 2 * ext[1:-1] + ext[:-2]) / h ** 2
+    return np.moveaxis(d, 0, axis)
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1), (3, 3, 3), (0, 1, 3), (2, 4, 0), (4, 3, 2)])
+def test_19point_matches_explicit_stencil(bc):
+    """(L19 + lambda) u = B f on the unknown box, with L19 = sum D_a + sum_{a<b}
+    (h_a^2 + h_b^2)/12 D_a D_b and B = I + sum h_a^2/12 D_a applied in physical
+    space (homogeneous boundary data, unequal spacings): pins the spectral
+    construction of oracle_helmholtz19 independently of its transforms."""
+    rng = np.random.default_rng(19)
+    n, dom, lam = (8, 6, 10), (0.0, 1.0, 0.0, 0.8, -0.5, 0.75), -1.7
+    h = [(dom[2 * a + 1] - dom[2 * a]) / n[a] for a in range(3)]
+    f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+    # homogeneous Dirichlet values, zero Neumann data
+    if bc[0] in (1, 2): f[:, :, 0] = 0
+    if bc[0] in (1, 4): f[:, :, -1] = 0
+    if bc[1] in (1, 2): f[:, 0, :] = 0
+    if bc[1] in (1, 4): f[:, -1, :] = 0
+    if bc[2] in (1, 2): f[0] = 0
+    if bc[2] in (1, 4): f[-1] = 0
+    sizes = [(n[2] + 1) * (n[1] + 1)] * 2 + [(n[2] + 1) * (n[0] + 1)] * 2 + [(n[1] + 1) * (n[0] + 1)] * 2
+    bcs = [np.zeros(sizes[i]) if bc[i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None for i in range(6)]
+    u = O.helmholtz(f, n, bc, lam, bcs, dom, stencil=19)
+    win = [O.unknown_window(bc[a], n[a]) for a in range(3)]
+    box = (slice(win[2][0], win[2][1] + 1), slice(win[1][0], win[1][1] + 1), slice(win[0][0], win[0][1] + 1))
+    ub, fb = u[box], f[box]
+    ax = {0: 2, 1: 1, 2: 0}   # grid axis a -> numpy axis
+    D = lambda v, a: _closure_d2(v, ax[a], bc[a], h[a])
+    L = sum(D(ub, a) for a in range(3))
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        L = L + (h[a] ** 2 + h[b] ** 2) / 12 * D(D(ub, b), a)
+    Bf = fb + sum(h[a] ** 2 / 12 * D(fb, a) for a in range(3))
+    err = np.max(np.abs(L + lam * ub - Bf)) / np.max(np.abs(Bf))
+    assert err < 1e-12, (bc, err)
+    # and it is not the 7-point solution
+    assert np.max(np.abs(u - O.helmholtz(f, n, bc, lam, bcs, dom))) > 1e-3 * np.max(np.abs(u))
+
+
+def test_19point_fourth_order_poisson():
+    """lambda = 0, periodic, smooth zero-mean u: the error falls as h^4."""
+    errs = []
+    for N in (16, 32, 64):
+        x = np.arange(N + 1) / N
+        Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+        u = np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) * np.sin(4 * np.pi * Z)
+        f = -(4 + 4 + 16) * np.pi ** 2 * u
+        uh = O.helmholtz(f, (N, N, N), (0, 0, 0), 0.0, stencil=19)
+        errs.append(np.max(np.abs(uh - u)))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 13.5 < r1 < 16.5 and 15.2 < r2 < 16.5, errs   # 14.2, 15.5: h^4 (h^2 would give 4)
