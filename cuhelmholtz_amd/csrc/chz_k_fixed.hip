@@ -7,6 +7,7 @@ namespace chz {
 
 const FixedEntry *fixed_table_f64(int *n);
 const FixedEntry *line_table_f64(int *n);
+const FixedEntry *line_table_f32(int *n);
 const FixedEntry *fixed_table_f32(int *n);
 const FixedEntry *batch_fixed_table_64_0(int *n);
 const FixedEntry *batch_fixed_table_64_1(int *n);
@@ -27,12 +28,12 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
 {
     static const bool off = getenv("CHZ_NO_FIXED") != nullptr;
     if (off) return nullptr;
-    // FP64: the lean-I/O kernels first (CHZ_NO_LINE in the environment: the
-    // plain fixed-shape kernels)
+    // the lean-I/O kernels first (CHZ_NO_LINE in the environment: the plain
+    // fixed-shape kernels)
     static const bool no_line = getenv("CHZ_NO_LINE") != nullptr;
     int n = 0;
-    if (prec == 64 && !no_line && p && line_windows_ok(*p, N)) {
-        const FixedEntry *t = line_table_f64(&n);
+    if (!no_line && p && line_windows_ok(*p, N)) {
+        const FixedEntry *t = prec == 64 ? line_table_f64(&n) : line_table_f32(&n);
         for (int i = 0; i < n; i++) {
             const FixedEntry &e = t[i];
             if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;

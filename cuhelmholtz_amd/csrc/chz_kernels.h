@@ -125,8 +125,8 @@ struct FixedEntry {
     const void *fn;
 };
 // fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if
-// none (CHZ_NO_FIXED in the environment disables them); given the pass arguments, the FP64
-// kernels with lean tile I/O are preferred where the pass windows allow them
+// none (CHZ_NO_FIXED in the environment disables them); given the pass arguments, the kernels
+// with lean tile I/O are preferred where the pass windows allow them
 const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
                                const PassArgs *p = nullptr);
 // fixed-shape batched kernel (arguments as k_batch), nullptr if none

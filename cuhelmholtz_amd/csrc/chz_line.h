@@ -1,4 +1,4 @@
-// chz_line.h -- fixed-shape FP64 pass kernels with lean tile I/O (N = 512).
+// chz_line.h -- fixed-shape pass kernels with lean tile I/O (N = 512).
 //
 // The fold, FFT and unfold phases are those of the generic pass (chz_pass.h,
 // chz_device.h).  What differs is how a tile moves between HBM and LDS:
@@ -20,16 +20,18 @@
 namespace chz {
 namespace line {
 
-// Line permutation of point row k for the row passes: L = 8 swaps by
-// (k >> 1) & 7, so the 16 consecutive points of a row-staging instruction land
-// in 16 distinct 16-byte bank groups; L = 16 rows are 256 bytes and permute by
-// k & 15.  Column passes stage with lanes = lines, which is conflict-free
-// unpermuted, and keep the plain layout (no index arithmetic in the FFT).
-template <int L, bool SW> __device__ __forceinline__ constexpr int swz(int k)
+// Line permutation of point row k for the row passes, chosen so that the 32
+// lanes staging one row (16 consecutive points, both halves) hit 32 distinct
+// bank slots and so that a chunk step (16 points) leaves it unchanged:
+//   FP64 (16-byte points): L = 8 by (k >> 1) & 7, L = 16 by k & 15;
+//   FP32 (8-byte points):  L = 8 by (k >> 2) & 3, L = 16 by (k >> 1) & 7.
+// Column passes stage with lanes = lines, which is conflict-free unpermuted,
+// and keep the plain layout (no index arithmetic in the FFT).
+template <int L, bool SW, int ESZ> __device__ __forceinline__ constexpr int swz(int k)
 {
     if constexpr (!SW) return 0;
-    else if constexpr (L == 8) return (k >> 1) & 7;
-    else return k & (L - 1);
+    else if constexpr (ESZ == 8) return L == 8 ? (k >> 1) & 7 : k & (L - 1);
+    else return L == 8 ? (k >> 2) & 3 : (k >> 1) & 7;
 }
 
 template <typename T, int LINES, bool SW> struct STile {
@@ -37,7 +39,7 @@ template <typename T, int LINES, bool SW> struct STile {
     static constexpr int L = LINES;
     V *buf;
     int c;
-    __device__ __forceinline__ int idx(int k) const { return k * L + (c ^ swz<L, SW>(k)); }
+    __device__ __forceinline__ int idx(int k) const { return k * L + (c ^ swz<L, SW, (int)sizeof(T)>(k)); }
     __device__ __forceinline__ V &pt(int k) const { return buf[idx(k)]; }
     __device__ __forceinline__ T &re(int s) const { return reinterpret_cast<T *>(buf)[2 * idx(s >> 1) + (s & 1)]; }
 };
@@ -48,6 +50,7 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     static constexpr int L = LINES, M = N / 2, NS = N + 2;
     static constexpr int NCH = (NS + TPL - 1) / TPL;   // slot chunks of TPL per line
     static_assert(!SW || TPL % 32 == 0, "swizzled tiles: threads per line a multiple of 32 (chunk steps keep the row swizzle)");
+    static_assert(!SW || L == 8 || L == 16, "swizzled tiles: 8 or 16 lines");
     static_assert(TPL % 2 == 0, "threads per line: even (a chunk step keeps the slot parity)");
     // chunk i (slots i*TPL .. i*TPL + TPL - 1) lies inside every window
     static constexpr bool interior(int i) { return i * TPL >= 2 && i * TPL + TPL - 1 <= N - 2; }
@@ -58,7 +61,7 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     // (a multiple of 16), which leaves the row swizzle unchanged
     __device__ __forceinline__ static T *lds_base(V *buf, int c, int s0)
     {
-        return reinterpret_cast<T *>(buf) + 2 * ((s0 >> 1) * L + (c ^ swz<L, SW>(s0 >> 1))) + (s0 & 1);
+        return reinterpret_cast<T *>(buf) + 2 * ((s0 >> 1) * L + (c ^ swz<L, SW, (int)sizeof(T)>(s0 >> 1))) + (s0 & 1);
     }
     static constexpr int lds_step = TPL * L;   // doubles per chunk
 
@@ -152,7 +155,134 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         }
     }
 
+    // refinement pass 1 of the mixed solve: FP64 residual r = f_fold - A U at
+    // the slots of rows (j, k) of the unknown box (body_xres of the generic
+    // pass), rounded to T.  Interior chunks load f, U and its four y / z
+    // neighbour rows at per-lane row pointers plus constant offsets, and the x
+    // neighbours at +-1 element; the x-boundary rules only enter the edge chunks.
+    __device__ __forceinline__ static void stage_xres(const SolveArgs &sa, const double *U, V *buf, int nlines,
+                                                      int a_tile, int k, int t)
+    {
+        constexpr int RC = 4;   // chunks whose loads are issued together
+        const int r = t / TPL, l = t % TPL;
+        const bool rok = r < nlines;
+        const int j = a_tile + (rok ? r : 0);
+        const long long fx = sa.NX + 1, fy = sa.NY + 1, fxy = fx * fy;
+        const long long c0 = (long long)a_tile * fx + (long long)k * fxy;   // wave-uniform
+        const unsigned rowoff = (unsigned)((rok ? r : 0) * (int)fx);
+        const double *fr = sa.f + c0 + rowoff, *ur = U + c0 + rowoff;        // row starts
+        const double *fl = fr + l, *ul = ur + l;                              // this lane's slot l
+        // y / z neighbour rows of U: offsets and 0 / 1 weights (periodic wrap, Neumann mirror, Dirichlet 0)
+        long long dm[2], dp[2];
+        double wm[2], wp[2];
+        const int q[2] = {j, k};
+        const long long st[2] = {fx, fxy};
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const int lo = sa.s[d + 1], hi = sa.e[d + 1], bc = sa.bc[d + 1];
+            dm[d] = 0; dp[d] = 0; wm[d] = 0.0; wp[d] = 0.0;
+            if (q[d] > lo) { dm[d] = -st[d]; wm[d] = 1.0; }
+            else if (bc == 0) { dm[d] = (long long)(hi - lo) * st[d]; wm[d] = 1.0; }
+            else if ((bc == 3 || bc == 4) && hi > lo) { dm[d] = st[d]; wm[d] = 1.0; }
+            if (q[d] < hi) { dp[d] = st[d]; wp[d] = 1.0; }
+            else if (bc == 0) { dp[d] = -(long long)(hi - lo) * st[d]; wp[d] = 1.0; }
+            else if ((bc == 2 || bc == 3) && hi > lo) { dp[d] = -st[d]; wp[d] = 1.0; }
+        }
+        const double *uym = ul + dm[0], *uyp = ul + dp[0], *uzm = ul + dm[1], *uzp = ul + dp[1];
+        const RowFolds rf(sa, j, k);
+        const int xs = sa.s[0], xe = sa.e[0], bx = sa.bc[0];
+        const int xm_end = bx == 0 ? xe : ((bx == 3 || bx == 4) && xe > xs ? xs + 1 : xs);
+        const double xm_w = (bx == 0 || ((bx == 3 || bx == 4) && xe > xs)) ? 1.0 : 0.0;
+        const int xp_end = bx == 0 ? xs : ((bx == 2 || bx == 3) && xe > xs ? xe - 1 : xe);
+        const double xp_w = (bx == 0 || ((bx == 2 || bx == 3) && xe > xs)) ? 1.0 : 0.0;
+        const double pert = sa.pertrb ? sa.pertrb[0] * sa.pert_scale : 0.0;
+        const double lam = sa.lambda, ix = sa.ih2[0], iy = sa.ih2[1], iz = sa.ih2[2];
+        T *lp = lds_base(buf, r, l);
+        static_for<0, (NCH + RC - 1) / RC>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            double fv[RC], u0[RC], um[RC], up[RC], uy[RC], uz[RC];
+            static_for<0, RC>([&](auto ic) {
+                constexpr int ii = decltype(ic)::value, i = g * RC + ii;
+                if constexpr (i < NCH) {
+                    if constexpr (interior(i)) {
+                        fv[ii] = fl[i * TPL];
+                        u0[ii] = ul[i * TPL];
+                        um[ii] = ul[i * TPL - 1];
+                        up[ii] = ul[i * TPL + 1];
+                        uy[ii] = wm[0] * uym[i * TPL] + wp[0] * uyp[i * TPL];
+                        uz[ii] = wm[1] * uzm[i * TPL] + wp[1] * uzp[i * TPL];
+                    } else {
+                        const int s = l + i * TPL;
+                        const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
+                        const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
+                        fv[ii] = fr[sc_];
+                        u0[ii] = ur[sc_];
+                        um[ii] = ur[sm];
+                        up[ii] = ur[sp];
+                        uy[ii] = wm[0] * (ur + dm[0])[sc_] + wp[0] * (ur + dp[0])[sc_];
+                        uz[ii] = wm[1] * (ur + dm[1])[sc_] + wp[1] * (ur + dp[1])[sc_];
+                    }
+                }
+            });
+            if (rf.any) {
+                static_for<0, RC>([&](auto ic) {
+                    constexpr int ii = decltype(ic)::value, i = g * RC + ii;
+                    if constexpr (i < NCH) {
+                        const int s = l + i * TPL, sc_ = (s >= xs && s <= xe) ? s : xs;
+#pragma unroll
+                        for (int m = 0; m < 4; m++)
+                            if (rf.fc[m] != 0.0) fv[ii] += rf.fc[m] * rf.fp[m][sc_];
+                    }
+                });
+            }
+            static_for<0, RC>([&](auto ic) {
+                constexpr int ii = decltype(ic)::value, i = g * RC + ii;
+                if constexpr (i < NCH) {
+                    const int s = l + i * TPL;
+                    const double a = u0[ii];
+                    double acc = fv[ii] - pert, m_ = um[ii], p_ = up[ii];
+                    bool ok = rok;
+                    if constexpr (!interior(i)) {
+                        acc += (s == xs ? rf.xlo : 0.0) + (s == xe ? rf.xhi : 0.0);
+                        m_ *= s > xs ? 1.0 : xm_w;
+                        p_ *= s < xe ? 1.0 : xp_w;
+                        ok = ok && s >= xs && s <= xe;
+                    }
+                    double au = lam * a + (m_ - 2 * a + p_) * ix;
+                    au += (uy[ii] - 2 * a) * iy;
+                    au += (uz[ii] - 2 * a) * iz;
+                    if (s < NS) lp[i * lds_step] = ok ? (T)(acc - au) : (T)0;
+                }
+            });
+        });
+    }
+
     // ---- store ---------------------------------------------------------------
+    // last pass of a mixed-precision solve (body_xupd of the generic pass):
+    // out = uin + scale * delta in FP64 over slots [st_lo, st_hi] of every row
+    // (uin == nullptr: out = scale * delta)
+    template <bool FULL>
+    __device__ __forceinline__ static void store_rows_upd(const PassArgs &p, V *buf, int nlines, const double *uin,
+                                                          double *out, long long obase, int t, bool negodd)
+    {
+        const int r = t / TPL, l = t % TPL;
+        if (!FULL && r >= nlines) return;
+        const double sc = (negodd && (l & 1)) ? -p.scale : p.scale;
+        const int hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
+        const unsigned lane = (unsigned)(r * (int)p.d_sa + l);
+        double *orow = out + obase + lane;
+        const double *irow = uin ? uin + obase + lane : nullptr;
+        const T *lp = lds_base(buf, r, l);
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            const int s = l + i * TPL;
+            if (s >= NS) continue;
+            if (!interior(i) && !(s >= p.st_lo && s <= hi)) continue;
+            double v = (double)lp[i * lds_step] * sc;
+            if (irow) v += irow[i * TPL];
+            orow[i * TPL] = v;
+        }
+    }
     // slots [st_lo, st_hi] of every line, value * scale inside [out_lo, out_hi], 0 outside
     // negodd: the C2R unfold (odd slots negated: the real view of conj(FFT)),
     // applied here instead of in LDS
@@ -217,9 +347,10 @@ __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, 
 // One pass of the solve or the transform on a tile (op as BatchOp: rows,
 // columns, solve pass 1, fused z pass).
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
-__device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int a_tile, int b, int nlines, unsigned char *smem)
+__device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
+                                    int b, int nlines, unsigned char *smem)
 {
-    typedef Io<T, N, TPL, L, OP == B_ROWS || OP == B_XFWD> IO;
+    typedef Io<T, N, TPL, L, OP != B_COLS && OP != B_ZMID> IO;
     typedef typename IO::V V;
     typedef typename IO::TL TL;
     constexpr int M = N / 2;
@@ -232,9 +363,10 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int 
     const TL tl{buf, c};
     stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
     const T *src = reinterpret_cast<const T *>(p.src) + (long long)a_tile * p.s_sa + (long long)b * p.s_sb;
-    if constexpr (OP == B_ROWS) IO::template stage_rows<FULL>(p, buf, nlines, src, t);
+    if constexpr (OP == B_ROWS || OP == B_XUPD) IO::template stage_rows<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_COLS || OP == B_ZMID) IO::template stage_cols<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_XFWD) IO::stage_xfwd(sa, buf, nlines, a_tile, b, t);
+    if constexpr (OP == B_XRES) IO::stage_xres(sa, u_in, buf, nlines, a_tile, b, t);
     __syncthreads();
     PassArgs q = p;
     // the last transform of the pass; a C2R one leaves its unfold (a sign
@@ -269,8 +401,11 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, int 
             line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tw, twl, red);
         }
     }
-    T *dst = reinterpret_cast<T *>(p.dst) + (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
-    if constexpr (OP == B_ROWS || OP == B_XFWD) IO::template store_rows<FULL>(q, buf, nlines, dst, t, last == K_C2R);
+    const long long dbase = (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
+    T *dst = reinterpret_cast<T *>(p.dst) + dbase;
+    if constexpr (OP == B_XUPD) IO::template store_rows_upd<FULL>(q, buf, nlines, u_in, u_out, dbase, t, last == K_C2R);
+    else if constexpr (OP == B_ROWS || OP == B_XFWD || OP == B_XRES)
+        IO::template store_rows<FULL>(q, buf, nlines, dst, t, last == K_C2R);
     else IO::template store_cols<FULL>(q, buf, nlines, dst, t, last == K_C2R);
 }
 
@@ -291,7 +426,7 @@ template <typename T, int N, int TPL, int L> constexpr int line_occupancy()
 // slots 2..N-2, which holds for every kind and solve pass.
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
 __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L>())) k_line(PassArgs p, SolveArgs sa,
-                                                                                  const double *, double *)
+                                                                                  const double *u_in, double *u_out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     PassArgs q = p;
@@ -303,8 +438,8 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L>())) k_li
     int a_tile, b;
     tile_of(q, a_tile, b);
     const int nlines = min(L, q.a0 + q.na - a_tile);
-    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, sa, a_tile, b, nlines, smem);
-    else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, a_tile, b, nlines, smem);
+    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, sa, u_in, u_out, a_tile, b, nlines, smem);
+    else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, u_in, u_out, a_tile, b, nlines, smem);
 }
 
 }  // namespace chz
