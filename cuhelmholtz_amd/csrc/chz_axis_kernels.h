@@ -241,23 +241,28 @@ __device__ __forceinline__ void zmid_divide(const PassArgs &p, const SolveArgs &
 // the point pairs (k, M - k), so they run as one pass: read Z_k, Z_{M-k},
 // form X_k, X_{M-k}, scale, recombine into the (conjugated) inverse-FFT input.
 // Same arithmetic as unfold<K_R2C> -> zmid_divide -> fold<K_C2R>.
-template <int EPT, typename T, typename TL, bool S19 = false>
+// w^k is read as wsrc[WST * k] (the pass's global table: WST = 4, a per-tile
+// copy of w^0..w^(M/2): WST = 1), the z eigenvalue of slot s as ezsrc[s]
+// (null: the global table of sa).
+template <int EPT, typename T, typename TL, bool S19 = false, int WST = 4>
 __device__ __forceinline__ void periodic_split_divide(const TL &tl, int u, int TPL, int M, int nlines, int a_tile,
                                                       int y, const SolveArgs &sa,
-                                                      const typename vec2_of<T>::type *__restrict__ tw)
+                                                      const typename vec2_of<T>::type *__restrict__ wsrc,
+                                                      const double *__restrict__ ezsrc = nullptr)
 {
     typedef typename vec2_of<T>::type V;
     constexpr int PP = EPT / 2 + 1;
     const T h = (T)0.5;
     const int c = tl.c, x = a_tile + c;
-    const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
+    const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2);
+    const double *ez = ezsrc ? ezsrc : ey + (sa.NY + 2);
     const LineDivide<S19> dv(sa, c < nlines ? ex[x] : 0.0, c < nlines ? ey[y] : 0.0);
     const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
     V ca[PP], cb[PP], wk[PP];
 #pragma unroll
     for (int i = 0; i < PP; i++) {
         const int k = u + i * TPL;
-        wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);   // w^k, w = e^{2 pi i / N}
+        wk[i] = k <= M / 2 ? wsrc[WST * k] : mk<V>(0, 0);   // w^k, w = e^{2 pi i / N}
     }
 #pragma unroll
     for (int i = 0; i < PP; i++) {

@@ -7,7 +7,7 @@ namespace chz {
 
 constexpr unsigned long long L256_884_32 = 0x040808ull;
 
-#define LN32(OP, L) {32, OP, 512, 8, 32, L, L256_884_32, (const void *)k_line<float, OP, 512, 32, L, L256_884_32, 8>}
+#define LN32(OP, L) {32, OP, 512, 8, 32, L, L256_884_32, (const void *)k_line<float, OP, 512, 32, L, L256_884_32, 8>, line_lds_extra<float, 512, OP>()}
 
 static const FixedEntry kLine32[] = {
     LN32(B_ROWS, 8), LN32(B_ROWS, 16), LN32(B_COLS, 8), LN32(B_COLS, 16),

@@ -8,13 +8,15 @@ namespace chz {
 constexpr unsigned long long L256_884 = 0x040808ull;
 constexpr unsigned long long L256_1616 = 0x1010ull;
 
-#define LN(OP, L) {64, OP, 512, 8, 32, L, L256_884, (const void *)k_line<double, OP, 512, 32, L, L256_884, 8>}
+#define LN(OP, L) {64, OP, 512, 8, 32, L, L256_884, (const void *)k_line<double, OP, 512, 32, L, L256_884, 8>, \
+                   line_lds_extra<double, 512, OP>()}
 
 static const FixedEntry kLine64[] = {
     LN(B_ROWS, 8), LN(B_ROWS, 16), LN(B_COLS, 8), LN(B_COLS, 16),
     LN(B_XFWD, 8), LN(B_XFWD, 16), LN(B_ZMID, 8), LN(B_ZMID, 16),
-    {64, B_COLS, 512, 16, 16, 16, L256_1616, (const void *)k_line<double, B_COLS, 512, 16, 16, L256_1616, 16>},
-    {64, B_ZMID, 512, 16, 16, 16, L256_1616, (const void *)k_line<double, B_ZMID, 512, 16, 16, L256_1616, 16>},
+    {64, B_COLS, 512, 16, 16, 16, L256_1616, (const void *)k_line<double, B_COLS, 512, 16, 16, L256_1616, 16>, 0u},
+    {64, B_ZMID, 512, 16, 16, 16, L256_1616, (const void *)k_line<double, B_ZMID, 512, 16, 16, L256_1616, 16>,
+     line_lds_extra<double, 512, B_ZMID>()},
 };
 
 const FixedEntry *line_table_f64(int *n)

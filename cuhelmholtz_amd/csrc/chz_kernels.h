@@ -123,6 +123,7 @@ struct FixedEntry {
     int prec, op, N, ept, tpl, lines;
     unsigned long long plan;
     const void *fn;
+    unsigned lds_extra;   // LDS bytes beyond axis_pass_lds (tables the kernel stages)
 };
 // fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if
 // none (CHZ_NO_FIXED in the environment disables them); given the pass arguments, the kernels

@@ -15,7 +15,7 @@ size_t axis_pass_lds(int prec, int M, int L, int TPL)
 hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
                         int prec, int nb, hipStream_t s)
 {
-    const size_t lds = axis_pass_lds(prec, p.M, p.L, p.TPL);
+    const size_t lds = axis_pass_lds(prec, p.M, p.L, p.TPL) + fx->lds_extra;
     dim3 grid((unsigned)(((p.na + p.L - 1) / p.L) * (long long)nb), 1, 1);
     const void *fn = fx->fn;
     PassArgs a = p;

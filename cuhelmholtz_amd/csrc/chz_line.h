@@ -328,11 +328,11 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
 // the divide of the fused z pass (zmid_divide of the generic pass) on the swizzled tile
 template <typename T, bool S19, typename TLT>
 __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, const TLT &tl, int u, int TPL, int nlines,
-                                        int a_tile, int y)
+                                        int a_tile, int y, const double *__restrict__ ez)
 {
     const int c = tl.c;
     const int x = a_tile + c;
-    const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2), *ez = ey + (sa.NY + 2);
+    const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2);
     const LineDivide<S19> dv(sa, c < nlines ? ex[x] : 0.0, c < nlines ? ey[y] : 0.0);
     const bool m0xy = sa.singular && c < nlines && (sa.bc[0] == 0 ? x < 2 : x == 0) && (sa.bc[1] == 0 ? y < 2 : y == 0);
 #pragma unroll 4
@@ -362,6 +362,15 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const TL tl{buf, c};
     stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
+    // fused z pass: the z eigenvalues and w^0..w^(M/2) of the divide, staged
+    // with the tile (their loads overlap the tile's instead of following a barrier)
+    double *ezl = reinterpret_cast<double *>(red + TPL * L);
+    V *wkl = reinterpret_cast<V *>(ezl + (N + 2));
+    if constexpr (OP == B_ZMID) {
+        const double *ezg = sa.eig + (sa.NX + 2) + (sa.NY + 2);
+        for (int i = t; i < N + 2; i += TPL * L) ezl[i] = ezg[i];
+        for (int i = t; i <= M / 2; i += TPL * L) wkl[i] = tw[4 * i];
+    }
     const T *src = reinterpret_cast<const T *>(p.src) + (long long)a_tile * p.s_sa + (long long)b * p.s_sb;
     if constexpr (OP == B_ROWS || OP == B_XUPD) IO::template stage_rows<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_COLS || OP == B_ZMID) IO::template stage_cols<FULL>(p, buf, nlines, src, t);
@@ -376,13 +385,13 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         last = sa.kind2;
         if (p.kind == K_R2C && sa.kind2 == K_C2R) {
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
-            if (sa.stencil19) periodic_split_divide<EPT, T, TL, true>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
-            else periodic_split_divide<EPT, T, TL, false>(tl, u, TPL, M, nlines, a_tile, b, sa, tw);
+            if (sa.stencil19) periodic_split_divide<EPT, T, TL, true, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
+            else periodic_split_divide<EPT, T, TL, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
         } else {
             line_transform<EPT, T>(p.kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
-            if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b);
-            else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b);
+            if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
+            else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
             __syncthreads();
             if (last == K_C2R) {
                 fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
@@ -413,9 +422,17 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
 
 // Waves per SIMD that the LDS of one tile leaves room for (160 KB per CU, at
 // most 4): the register budget of the kernel follows from it.
-template <typename T, int N, int TPL, int L> constexpr int line_occupancy()
+// LDS bytes a kernel stages beyond the tile, FFT twiddles and reduction
+// scratch: the fused z pass keeps the z eigenvalues and w^0..w^(M/2)
+template <typename T, int N, int OP> constexpr unsigned line_lds_extra()
 {
-    constexpr long lds = ((long)(N / 2 + 1) * L + N / 2) * 2 * (long)sizeof(T) + (long)TPL * L * (long)sizeof(T);
+    return OP == B_ZMID ? (unsigned)((N + 2) * sizeof(double) + (N / 4 + 1) * 2 * sizeof(T)) : 0u;
+}
+
+template <typename T, int N, int TPL, int L, int OP> constexpr int line_occupancy()
+{
+    constexpr long lds = ((long)(N / 2 + 1) * L + N / 2) * 2 * (long)sizeof(T) + (long)TPL * L * (long)sizeof(T) +
+                         (long)line_lds_extra<T, N, OP>();
     constexpr long wgs = 163840 / lds;
     constexpr long w = wgs * (TPL * L / 64) / 4;
     return w < 1 ? 1 : w > 4 ? 4 : (int)w;
@@ -425,7 +442,7 @@ template <typename T, int N, int TPL, int L> constexpr int line_occupancy()
 // Precondition (checked by the host): every window of the pass contains
 // slots 2..N-2, which holds for every kind and solve pass.
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
-__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L>())) k_line(PassArgs p, SolveArgs sa,
+__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) k_line(PassArgs p, SolveArgs sa,
                                                                                   const double *u_in, double *u_out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
