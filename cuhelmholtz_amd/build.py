@@ -20,9 +20,12 @@ BIN = os.path.join(HERE, "bin")
 ARCH = os.environ.get("CHZ_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
-# packed FP32 instructions are disabled for all device code
+# packed FP32 instructions are disabled for all device code; multiply-adds are
+# fused within a source expression only (-ffp-contract=on), so a tile body
+# rounds the same way in every kernel it is inlined into (a batched problem
+# and its single solve give the same bits)
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics", "-ffp-contract=on"]
 
 
 def hipcc():

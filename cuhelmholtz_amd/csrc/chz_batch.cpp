@@ -265,14 +265,23 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         const Group &gr = groups[gi];
         const PassSpec &q = *gr.first;
         BatchLaunch l;
-        l.fn = batch_fixed_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+        l.lds = axis_pass_lds(q.prec, q.d.p.M, q.d.p.L, q.d.p.TPL);
+        // lean tile I/O where the windows of every problem of the group allow it
+        const FixedEntry *ln = batch_line_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+        for (size_t k = 0; ln && k < gr.probs.size(); k++)
+            if (!line_windows_ok(specs[gr.probs[k]][gr.step].d.p, q.N)) ln = nullptr;
+        if (ln) {
+            l.fn = ln->fn;
+            l.lds += ln->lds_extra;
+        } else {
+            l.fn = batch_fixed_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+        }
         if (!l.fn) l.fn = q.prec == 64 ? batch_kernel_f64(q.op, q.ept) : batch_kernel_f32(q.op, q.ept);
         if (!l.fn) {
             delete plan;
             return fail(ERR_UNSUPPORTED, "no batched kernel for this pass");
         }
         l.threads = q.d.p.L * q.d.p.TPL;
-        l.lds = axis_pass_lds(q.prec, q.d.p.M, q.d.p.L, q.d.p.TPL);
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l.fn, l.threads, l.lds) != hipSuccess || occ <= 0) occ = 1;
         long long grid = std::min<long long>(nitems[gi], (long long)occ * cus);

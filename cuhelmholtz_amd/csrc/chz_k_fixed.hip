@@ -8,16 +8,23 @@ namespace chz {
 const FixedEntry *fixed_table_f64(int *n);
 const FixedEntry *line_table_f64(int *n);
 const FixedEntry *line_table_f32(int *n);
+const FixedEntry *line_small_table_f64(int *n);
+const FixedEntry *line_small_table_f32(int *n);
 const FixedEntry *fixed_table_f32(int *n);
 const FixedEntry *batch_fixed_table_64_0(int *n);
 const FixedEntry *batch_fixed_table_64_1(int *n);
 const FixedEntry *batch_fixed_table_32_0(int *n);
 const FixedEntry *batch_fixed_table_32_1(int *n);
 const FixedEntry *batch_fixed_table_32_2(int *n);
+const FixedEntry *batch_line_table_64_0(int *n);
+const FixedEntry *batch_line_table_64_1(int *n);
+const FixedEntry *batch_line_table_32_0(int *n);
+const FixedEntry *batch_line_table_32_1(int *n);
+const FixedEntry *batch_line_table_32_2(int *n);
 
 // The lean-I/O kernels check the input, store and output windows only in the
 // first and last slot chunks of a line: every window must contain 2..N-2.
-static bool line_windows_ok(const PassArgs &p, int N)
+bool line_windows_ok(const PassArgs &p, int N)
 {
     auto in = [N](int lo, int hi) { return lo <= 2 && hi >= N - 2; };
     return in(p.in_lo, p.in_hi) && in(p.st_lo, p.st_hi) && in(p.out_lo, p.out_hi);
@@ -33,10 +40,13 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
     static const bool no_line = getenv("CHZ_NO_LINE") != nullptr;
     int n = 0;
     if (!no_line && p && line_windows_ok(*p, N)) {
-        const FixedEntry *t = prec == 64 ? line_table_f64(&n) : line_table_f32(&n);
-        for (int i = 0; i < n; i++) {
-            const FixedEntry &e = t[i];
-            if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
+        for (int q = 0; q < 2; q++) {
+            const FixedEntry *t = prec == 64 ? (q ? line_small_table_f64(&n) : line_table_f64(&n))
+                                             : (q ? line_small_table_f32(&n) : line_table_f32(&n));
+            for (int i = 0; i < n; i++) {
+                const FixedEntry &e = t[i];
+                if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
+            }
         }
     }
     const FixedEntry *t = prec == 64 ? fixed_table_f64(&n) : fixed_table_f32(&n);
@@ -62,6 +72,26 @@ const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
         for (int i = 0; i < n; i++) {
             const FixedEntry &e = t[i];
             if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return e.fn;
+        }
+    }
+    return nullptr;
+}
+
+const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
+{
+    static const bool off = getenv("CHZ_NO_FIXED") != nullptr || getenv("CHZ_NO_LINE") != nullptr;
+    if (off) return nullptr;
+    typedef const FixedEntry *(*Tab)(int *);
+    static const Tab t64[] = {batch_line_table_64_0, batch_line_table_64_1};
+    static const Tab t32[] = {batch_line_table_32_0, batch_line_table_32_1, batch_line_table_32_2};
+    const Tab *tabs = prec == 64 ? t64 : t32;
+    const int ntab = prec == 64 ? 2 : 3;
+    for (int q = 0; q < ntab; q++) {
+        int n = 0;
+        const FixedEntry *t = tabs[q](&n);
+        for (int i = 0; i < n; i++) {
+            const FixedEntry &e = t[i];
+            if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
         }
     }
     return nullptr;

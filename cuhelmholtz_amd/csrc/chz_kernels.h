@@ -132,6 +132,12 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
                                const PassArgs *p = nullptr);
 // fixed-shape batched kernel (arguments as k_batch), nullptr if none
 const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+// batched kernel with lean tile I/O (arguments as k_batch), nullptr if none
+// (CHZ_NO_LINE / CHZ_NO_FIXED in the environment: none); usable when every
+// problem of the launch passes line_windows_ok
+const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+// the windows of a pass contain slots 2..N-2 (precondition of the lean-I/O kernels)
+bool line_windows_ok(const PassArgs &p, int N);
 hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
                         int prec, int nb, hipStream_t s);
 

@@ -1,4 +1,4 @@
-// chz_line.h -- fixed-shape pass kernels with lean tile I/O (N = 512).
+// chz_line.h -- fixed-shape pass kernels with lean tile I/O.
 //
 // The fold, FFT and unfold phases are those of the generic pass (chz_pass.h,
 // chz_device.h).  What differs is how a tile moves between HBM and LDS:
@@ -348,13 +348,14 @@ __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, 
 // columns, solve pass 1, fused z pass).
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
-                                    int b, int nlines, unsigned char *smem)
+                                    int b, int nlines, unsigned char *smem, int t)
 {
-    typedef Io<T, N, TPL, L, OP != B_COLS && OP != B_ZMID> IO;
+    // row tiles are swizzled where the chunk step allows it (32k threads per line, 8 or 16 lines)
+    constexpr bool SW = OP != B_COLS && OP != B_ZMID && TPL % 32 == 0 && (L == 8 || L == 16);
+    typedef Io<T, N, TPL, L, SW> IO;
     typedef typename IO::V V;
     typedef typename IO::TL TL;
     constexpr int M = N / 2;
-    const int t = (int)threadIdx.x;
     const int c = t % L, u = t / L;
     V *buf = reinterpret_cast<V *>(smem);
     V *twl = buf + (M + 1) * L;
@@ -455,8 +456,52 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) 
     int a_tile, b;
     tile_of(q, a_tile, b);
     const int nlines = min(L, q.a0 + q.na - a_tile);
-    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, sa, u_in, u_out, a_tile, b, nlines, smem);
-    else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, u_in, u_out, a_tile, b, nlines, smem);
+    const int t = (int)threadIdx.x;
+    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
+    else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
+}
+
+// Batched pass with lean tile I/O: the persistent work-item loop of the
+// batched passes (k_batch_fixed: items cut into 8 contiguous chunks, one per
+// XCD) around the tile body of k_line.  Same precondition on the windows of
+// every problem of the launch.
+template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
+__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
+    k_line_batch(const BatchItem *__restrict__ items, int nitems, const BatchDesc *__restrict__ descs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int G = (int)gridDim.x, bid = (int)blockIdx.x;
+    int w, step, end;
+    if (G >= 8 && G % 8 == 0) {
+        const int per = (nitems + 7) / 8, x = bid % 8;
+        w = x * per + bid / 8;
+        step = G / 8;
+        end = min(nitems, (x + 1) * per);
+    } else {
+        w = bid;
+        step = G;
+        end = nitems;
+    }
+    for (; w < end; w += step) {
+        // a thread index the compiler cannot hoist address arithmetic out of
+        // the item loop with (which would hold it in registers across items)
+        int t = (int)threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const BatchItem it = items[w];
+        const BatchDesc *d = descs + it.prob;
+        PassArgs q = d->p;
+        q.N = N;
+        q.M = N / 2;
+        q.TPL = TPL;
+        q.L = L;
+        q.plan = PLAN;
+        const int ntiles = (q.na + L - 1) / L;
+        const int a_tile = q.a0 + (it.tile % ntiles) * L, b = q.b0 + it.tile / ntiles;
+        const int nlines = min(L, q.a0 + q.na - a_tile);
+        if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t);
+        else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t);
+        __syncthreads();   // the next item reuses the tile memory
+    }
 }
 
 }  // namespace chz
