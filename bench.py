@@ -26,7 +26,10 @@ Prints ONE JSON line (rank 0):
               FP32 + refinement, against the same problems solved one by one
   config.adjoint  forward and adjoint backward of the differentiable solve
               (batch 16 of 128^3, codes (1,3,0), lambda = -2)
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-adjoint]
+  config.fp64  also: per-step median, the first call's time (workspace and
+              table setup included) and an end-to-end solve from host arrays
+              through hw3crt_ (allocation and PCIe copies included)
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-adjoint] [--no-e2e]
 """
 import argparse
 import json
@@ -237,6 +240,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, nargs=3, default=[512, 512, 512])
     ap.add_argument("--no-batch", action="store_true", help="skip the 2048-problem batched solve")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end hw3crt_ timing from host arrays")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-solve timing")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -260,10 +264,15 @@ def main():
     del f_np
     bct = [None if b is None else torch.from_numpy(b).to(dev) for b in bcs]
 
-    # accuracy of the measured path (FP64 plain: normalised 7-point residual)
+    # accuracy of the measured path (FP64 plain: normalised 7-point residual);
+    # this first call also creates the solver workspace and tables (setup time)
     u = f0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     C.helmholtz_solve(u, n, bc, lam, bct)
     torch.cuda.synchronize()
+    setup_ms = 1e3 * (time.perf_counter() - t0)
+    log(f"fp64 first solve (workspace + table setup included): {setup_ms:.1f} ms")
     rres = checks.relative_residual(u, f0, n, bc, lam, bcs)
     del u
     torch.cuda.empty_cache()
@@ -275,16 +284,17 @@ def main():
             solve_fn(ft)
         torch.cuda.synchronize()
         C.set_timing(True)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        e0.record()
-        for _ in range(args.steps):
+        ev[0].record()
+        for i in range(args.steps):
             solve_fn(ft)
-        e1.record()
+            ev[i + 1].record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        ms = e0.elapsed_time(e1) / args.steps
+        ms = ev[0].elapsed_time(ev[-1]) / args.steps
+        med = float(np.median([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]))
         passes = C.get_timing(64 * args.steps)
         C.set_timing(False)
         del ft
@@ -295,10 +305,11 @@ def main():
         avg = {k: float(np.mean(v)) for k, v in per.items()}
         for k, v in avg.items():
             log(f"  {label} pass {k:8s} {v:8.3f} ms x{len(per[k]) // args.steps}")
-        return ms, avg, wall
+        log(f"  {label} per-step median {med:.3f} ms over {args.steps} steps")
+        return ms, avg, wall, med
 
     # plain FP64
-    ms, avg, wall = timed(lambda t: C.helmholtz_solve(t, n, bc, lam, bct), "fp64")
+    ms, avg, wall, med = timed(lambda t: C.helmholtz_solve(t, n, bc, lam, bct), "fp64")
     dom = max((k for k in avg if k != "periodic"), key=lambda k: avg[k])
     achieved = 16.0 * V / avg[dom] / 1e6   # GB/s: one FP64 read + write per cell
     solves = 1000.0 / ms
@@ -322,11 +333,11 @@ def main():
     mixed_info = None
     if mixed is not None:
         k, err = mixed
-        mms, mavg, _ = timed(lambda t: C.helmholtz_solve_mixed(t, n, bc, lam, bct, refine=k), "mixed")
+        mms, mavg, _, mmed = timed(lambda t: C.helmholtz_solve_mixed(t, n, bc, lam, bct, refine=k), "mixed")
         mbytes = (48.0 + 64.0 * k) * V
         mixed_info = {"refine": k, "rel_err_vs_fp64": err, "ms": mms, "solves_per_s": 1000.0 / mms,
                       "hbm_gbs_model": mbytes / mms / 1e6, "bytes_per_cell": 48 + 64 * k,
-                      "vs_fp64": (1000.0 / mms) / solves, "pass_ms": mavg}
+                      "vs_fp64": (1000.0 / mms) / solves, "median_ms": mmed, "pass_ms": mavg}
         log(f"mixed solve (k={k}) {mms:.3f} ms -> {1000 / mms:.1f} solves/s ({mixed_info['vs_fp64']:.2f}x fp64), "
             f"{mbytes / mms / 1e6:.0f} GB/s model")
 
@@ -345,6 +356,21 @@ def main():
     tms = e0.elapsed_time(e1) / 5
     log(f"fp64 do_transform {kinds}: {tms:.3f} ms -> {1000 / tms:.0f} transforms/s, {48.0 * V / tms / 1e6:.0f} GB/s model")
     del x, y
+
+    # end to end from host arrays, as the reference's drivers time it: the
+    # hw3crt_ entry point (device allocation, host-to-device copy, solve,
+    # copy back); a reported figure, never the headline value
+    e2e_ms = None
+    if not args.no_e2e:
+        fh = f0.cpu().numpy()
+        bh = [None if b is None else b.cpu().numpy() for b in bct]
+        torch.cuda.empty_cache()
+        fw = fh.copy()
+        t0 = time.perf_counter()
+        _, ierr = C.hw3crt(fw, n, bc, lam, bh)
+        e2e_ms = 1e3 * (time.perf_counter() - t0)
+        log(f"hw3crt_ end to end (host arrays, allocation and PCIe copies included): {e2e_ms:.1f} ms, IERROR {ierr}")
+        del fh, fw, bh
 
     batched = None if args.no_batch else bench_batched(dev, max(1, min(args.steps, 10)))
     adjoint = None if args.no_adjoint else bench_adjoint(dev)
@@ -375,8 +401,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Helmholtz solve {n[0]}x{n[1]}x{n[2]}, codes (3,4,0), lambda -4, smooth modes + noise",
                    "path": best_path,
-                   "fp64": {"solves_per_s": solves, "ms": ms, "relative_residual": rres,
-                            "hbm_gbs_model": 80.0 * V / ms / 1e6, "pass_ms": avg},
+                   "fp64": {"solves_per_s": solves, "ms": ms, "median_ms": med, "relative_residual": rres,
+                            "hbm_gbs_model": 80.0 * V / ms / 1e6, "setup_first_call_ms": setup_ms,
+                            "hw3crt_end_to_end_ms": e2e_ms, "pass_ms": avg},
                    "fp32_refine": mixed_info,
                    "transforms_per_s_fp64": 1000.0 / tms,
                    "batched": batched,

@@ -243,8 +243,16 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         for (size_t k = 0; k < gr.probs.size(); k++) {
             const PassSpec &q = specs[gr.probs[k]][gr.step];
             d[k] = q.d;
-            const int nt = ((q.d.p.na + q.d.p.L - 1) / q.d.p.L) * q.nb;
-            for (int t = 0; t < nt; t++) it[w++] = BatchItem{(int)k, t};
+            // tile t = a + ntiles * b; listed a-fastest, or b-fastest where the
+            // pass reads neighbour planes (the residual: consecutive items then
+            // share rows of U in L2)
+            const int ntiles = (q.d.p.na + q.d.p.L - 1) / q.d.p.L;
+            if (q.d.p.bfast) {
+                for (int a = 0; a < ntiles; a++)
+                    for (int b = 0; b < q.nb; b++) it[w++] = BatchItem{(int)k, a + ntiles * b};
+            } else {
+                for (int t = 0; t < ntiles * q.nb; t++) it[w++] = BatchItem{(int)k, t};
+            }
         }
     }
     for (int a = 0; a < 3; a++)
