@@ -16,8 +16,8 @@ static const FixedEntry kTab[] = {
     LB(double, 64, B_XFWD, 96, 6, 8, 16, 0x2020206ull),
     LB(double, 64, B_XFWD, 128, 8, 8, 16, 0x808ull),
     LB(double, 64, B_ZMID, 32, 4, 4, 32, 0x404ull),
-    LB(double, 64, B_ZMID, 48, 8, 4, 16, 0x308ull),
-    LB(double, 64, B_ZMID, 64, 8, 4, 16, 0x408ull),
+    LB(double, 64, B_ZMID, 48, 8, 4, 32, 0x308ull),
+    LB(double, 64, B_ZMID, 64, 8, 4, 32, 0x408ull),
     LB(double, 64, B_ZMID, 96, 6, 8, 8, 0x2020206ull),
     LB(double, 64, B_ZMID, 128, 8, 8, 16, 0x808ull)
 };

@@ -27,8 +27,8 @@ static const FixedEntry kTab[] = {
     LS(double, 64, B_XFWD, 96, 6, 8, 16, 0x2020206ull),
     LS(double, 64, B_XFWD, 128, 8, 8, 16, 0x808ull),
     LS(double, 64, B_ZMID, 32, 4, 4, 32, 0x404ull),
-    LS(double, 64, B_ZMID, 48, 8, 4, 16, 0x308ull),
-    LS(double, 64, B_ZMID, 64, 8, 4, 16, 0x408ull),
+    LS(double, 64, B_ZMID, 48, 8, 4, 32, 0x308ull),
+    LS(double, 64, B_ZMID, 64, 8, 4, 32, 0x408ull),
     LS(double, 64, B_ZMID, 96, 6, 8, 8, 0x2020206ull),
     LS(double, 64, B_ZMID, 128, 8, 8, 16, 0x808ull)
 };

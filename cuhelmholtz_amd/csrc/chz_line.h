@@ -348,7 +348,7 @@ __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, 
 // columns, solve pass 1, fused z pass).
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
-                                    int b, int nlines, unsigned char *smem, int t)
+                                    int b, int nlines, unsigned char *smem, int t, bool tables = true)
 {
     // row tiles are swizzled where the chunk step allows it (32k threads per line, 8 or 16 lines)
     constexpr bool SW = OP != B_COLS && OP != B_ZMID && TPL % 32 == 0 && (L == 8 || L == 16);
@@ -362,12 +362,14 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     T *red = reinterpret_cast<T *>(twl + M);
     const V *tw = reinterpret_cast<const V *>(p.tw);
     const TL tl{buf, c};
-    stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
+    // the FFT twiddles and the z pass's tables (unchanged while a workgroup
+    // stays on one problem: tables == false)
+    if (tables) stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
     // fused z pass: the z eigenvalues and w^0..w^(M/2) of the divide, staged
     // with the tile (their loads overlap the tile's instead of following a barrier)
     double *ezl = reinterpret_cast<double *>(red + TPL * L);
     V *wkl = reinterpret_cast<V *>(ezl + (N + 2));
-    if constexpr (OP == B_ZMID) {
+    if constexpr (OP == B_ZMID) if (tables) {
         const double *ezg = sa.eig + (sa.NX + 2) + (sa.NY + 2);
         for (int i = t; i < N + 2; i += TPL * L) ezl[i] = ezg[i];
         for (int i = t; i <= M / 2; i += TPL * L) wkl[i] = tw[4 * i];
@@ -482,12 +484,15 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
         step = G;
         end = nitems;
     }
+    int prev = -1;   // problem of the previous item (its tables are in LDS)
     for (; w < end; w += step) {
         // a thread index the compiler cannot hoist address arithmetic out of
         // the item loop with (which would hold it in registers across items)
         int t = (int)threadIdx.x;
         asm volatile("" : "+v"(t));
         const BatchItem it = items[w];
+        const bool tables = it.prob != prev;
+        prev = it.prob;
         const BatchDesc *d = descs + it.prob;
         PassArgs q = d->p;
         q.N = N;
@@ -498,8 +503,10 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
         const int ntiles = (q.na + L - 1) / L;
         const int a_tile = q.a0 + (it.tile % ntiles) * L, b = q.b0 + it.tile / ntiles;
         const int nlines = min(L, q.a0 + q.na - a_tile);
-        if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t);
-        else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t);
+        if (nlines == L)
+            line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t, tables);
+        else
+            line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t, tables);
         __syncthreads();   // the next item reuses the tile memory
     }
 }
