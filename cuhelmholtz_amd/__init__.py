@@ -66,6 +66,8 @@ def _declare_solver(L):
         L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
     if hasattr(L, "chz_helmholtz_adjoint"):
         L.chz_helmholtz_adjoint.argtypes = ([d, d, i, i] * 3) + [d, vp, vp, vp] + [vp] * 6 + [vp, vp]
+    if hasattr(L, "chz_helmholtz_adjoint_batch"):
+        L.chz_helmholtz_adjoint_batch.argtypes = ([d, d, i, i] * 3) + [i, vp, vp, vp, vp] + [vp] * 6 + [vp, vp]
 
 
 class _Problem(ctypes.Structure):
@@ -77,6 +79,8 @@ def _declare_batch(L):
     vp, i = ctypes.c_void_p, ctypes.c_int
     L.chz_batch_create.argtypes = [ctypes.POINTER(_Problem), i, i, i, ctypes.POINTER(vp)]
     L.chz_batch_solve.argtypes = [vp, i, vp]
+    L.chz_batch_rebind.argtypes = [vp, ctypes.POINTER(_Problem), i, vp]
+    L.chz_batch_solve_cached.argtypes = [ctypes.POINTER(_Problem), i, i, i, vp]
     L.chz_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_longlong)]
     L.chz_batch_destroy.argtypes = [vp]
@@ -248,6 +252,110 @@ def helmholtz_adjoint(u, gu, n, bc, lam, dom=(0.0, 1.0) * 3, faces=False, want_l
     return gf, gfaces, glam
 
 
+def _batch_faces(bcs, n, bc, B, dev):
+    """Face arrays of a batch of one geometry: each None, (size,) shared by
+    the batch, or (B, size).  Returns per face (base pointer, per-problem byte
+    stride) or None, and the tensors to keep alive."""
+    import torch
+    NX, NY, NZ = n
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    out, keep = [], []
+    for i, b in enumerate(list(bcs) + [None] * (6 - len(bcs))):
+        a = i // 2
+        need = bc[a] in ((3, 4) if i % 2 == 0 else (2, 3))
+        if b is None:
+            if need:
+                raise ValueError(f"boundary code {bc[a]} along axis {a} needs face array {i}")
+            out.append(None)
+            continue
+        t = (b if isinstance(b, torch.Tensor) else torch.as_tensor(b)).to(dev, torch.float64).contiguous()
+        if t.numel() == sizes[i]:
+            out.append((t.data_ptr(), 0))
+        elif t.numel() == B * sizes[i]:
+            out.append((t.data_ptr(), 8 * sizes[i]))
+        else:
+            raise ValueError(f"face array {i} must have {sizes[i]} values or {B} x {sizes[i]}")
+        keep.append(t)
+    return out, keep
+
+
+def helmholtz_solve_batch(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3):
+    """In-place FP64 solve of a batch of problems of one geometry: f is a
+    contiguous CUDA float64 tensor (B, NZ+1, NY+1, NX+1); lam one value or B
+    values; bcs as helmholtz_solve, each face array (size,) shared by the
+    batch or (B, size).  One batched solve (chz_batch_solve_cached: the plan
+    for this geometry is made once and rebound on later calls)."""
+    import torch
+    NX, NY, NZ = (int(v) for v in n)
+    if not f.is_cuda or f.dtype != torch.float64 or not f.is_contiguous() or f.dim() != 4:
+        raise ValueError("f must be a contiguous CUDA float64 tensor (B, NZ+1, NY+1, NX+1)")
+    if tuple(f.shape[1:]) != (NZ + 1, NY + 1, NX + 1):
+        raise ValueError("f must be shaped (B, NZ+1, NY+1, NX+1)")
+    B = int(f.shape[0])
+    lams = [float(v) for v in (lam if hasattr(lam, "__len__") else [lam] * B)]
+    if len(lams) != B:
+        raise ValueError("lam must be one value or one per problem")
+    faces, keep = _batch_faces(bcs, (NX, NY, NZ), bc, B, f.device)
+    nodes = (NX + 1) * (NY + 1) * (NZ + 1)
+    arr = (_Problem * B)()
+    for k in range(B):
+        q = arr[k]
+        for i in range(6):
+            q.domain[i] = float(dom[i])
+            q.faces[i] = None if faces[i] is None else faces[i][0] + k * faces[i][1]
+        for a, v in enumerate((NX, NY, NZ)):
+            q.N[a] = v
+            q.bc[a] = int(bc[a])
+        q.lam = lams[k]
+        q.f = f.data_ptr() + 8 * nodes * k
+    with torch.cuda.device(f.device):
+        _check(lib().chz_batch_solve_cached(arr, B, 64, 0, _stream(f)))
+    del keep
+    return f
+
+
+def helmholtz_adjoint_batch(u, gu, n, bc, lam, dom=(0.0, 1.0) * 3, faces=False, want_lambda=True):
+    """helmholtz_adjoint for a batch of one geometry (chz_helmholtz_adjoint_batch):
+    u, gu (B, NZ+1, NY+1, NX+1) contiguous CUDA float64 tensors, lam one value
+    or B values.  Returns (gf (B, ...), face gradients (6-list of (B, size),
+    None where the codes read no face data, or all None if faces=False),
+    glambda ((B,) tensor or None))."""
+    import torch
+    NX, NY, NZ = (int(v) for v in n)
+    shape = (NZ + 1, NY + 1, NX + 1)
+    for t in (u, gu):
+        if t is not None and (not t.is_cuda or t.dtype != torch.float64 or t.dim() != 4
+                              or tuple(t.shape[1:]) != shape or not t.is_contiguous()):
+            raise ValueError("u and gu must be contiguous CUDA float64 tensors shaped (B, NZ+1, NY+1, NX+1)")
+    B = int(gu.shape[0])
+    if u is not None and int(u.shape[0]) != B:
+        raise ValueError("u and gu must hold the same number of problems")
+    lams = [float(v) for v in (lam if hasattr(lam, "__len__") else [lam] * B)]
+    if len(lams) != B:
+        raise ValueError("lam must be one value or one per problem")
+    lam_h = (ctypes.c_double * B)(*lams)
+    gf = torch.empty_like(gu)
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    gfaces = [None] * 6
+    if faces:
+        for i in range(6):
+            a = i // 2
+            if bc[a] in ((3, 4) if i % 2 == 0 else (2, 3)):
+                gfaces[i] = torch.empty((B, sizes[i]), dtype=torch.float64, device=gu.device)
+    glam = torch.empty(B, dtype=torch.float64, device=gu.device) if want_lambda else None
+    args = []
+    for a, N in enumerate((NX, NY, NZ)):
+        args += [float(dom[2 * a]), float(dom[2 * a + 1]), N, int(bc[a])]
+    ptr = [None if t is None else ctypes.c_void_p(t.data_ptr()) for t in gfaces]
+    with torch.cuda.device(gu.device):
+        _check(lib().chz_helmholtz_adjoint_batch(*args, B, ctypes.cast(lam_h, ctypes.c_void_p),
+                                                 None if u is None else ctypes.c_void_p(u.data_ptr()),
+                                                 ctypes.c_void_p(gu.data_ptr()), ctypes.c_void_p(gf.data_ptr()), *ptr,
+                                                 None if glam is None else ctypes.c_void_p(glam.data_ptr()),
+                                                 _stream(gu)))
+    return gf, gfaces, glam
+
+
 def set_timing(on):
     """Record per-pass events on every transform / solve (reference TIME_TEST counterpart)."""
     lib().chz_set_timing(1 if on else 0)
@@ -276,7 +384,18 @@ class BatchSolver:
 
     def __init__(self, problems, prec=64, refine=0):
         import torch
-        self._keep = []
+        arr, self._keep, dev = self._problems(problems)
+        self.device = dev if dev is not None else torch.device("cuda", torch.cuda.current_device())
+        self.count = len(problems)
+        self.prec, self.refine = prec, refine
+        self._plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(lib().chz_batch_create(arr, len(problems), int(prec), int(refine), ctypes.byref(self._plan)))
+
+    @staticmethod
+    def _problems(problems):
+        import torch
+        keep = []
         arr = (_Problem * len(problems))()
         dev = None
         for k, pr in enumerate(problems):
@@ -292,8 +411,8 @@ class BatchSolver:
                 dev = f.device
             elif f.device != dev:
                 raise ValueError("all problems must live on one device")
-            faces, keep = _faces(bcs, (NX, NY, NZ), bc, f.device)
-            self._keep += [f] + keep
+            faces, kf = _faces(bcs, (NX, NY, NZ), bc, f.device)
+            keep += [f] + kf
             q = arr[k]
             for i in range(6):
                 q.domain[i] = float(dom[i])
@@ -303,12 +422,7 @@ class BatchSolver:
                 q.bc[a] = int(bc[a])
             q.lam = float(lam)
             q.f = f.data_ptr()
-        self.device = dev
-        self.count = len(problems)
-        self.prec, self.refine = prec, refine
-        self._plan = ctypes.c_void_p()
-        with torch.cuda.device(dev):
-            _check(lib().chz_batch_create(arr, len(problems), int(prec), int(refine), ctypes.byref(self._plan)))
+        return arr, keep, dev
 
     def solve(self, graph=True):
         """Run the batched solve on the current stream (graph=False: direct
@@ -317,6 +431,16 @@ class BatchSolver:
         with torch.cuda.device(self.device):
             s = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
             _check(lib().chz_batch_solve(self._plan, 1 if graph else 0, s))
+
+    def rebind(self, problems):
+        """Point the plan at new problems of the same geometry (chz_batch_rebind):
+        new f / face tensors and lambdas; waits for the current stream."""
+        import torch
+        arr, keep, _ = self._problems(problems)
+        with torch.cuda.device(self.device):
+            s = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+            _check(lib().chz_batch_rebind(self._plan, arr, len(problems), s))
+        self._keep = keep
 
     def info(self):
         n, w, t = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()

@@ -1,6 +1,6 @@
 """Differentiable Helmholtz solve: torch.autograd.Function over the FP64
-solve (chz_helmholtz_solve) and its hand-written adjoint
-(chz_helmholtz_adjoint).
+solve (chz_helmholtz_solve; a batch: chz_batch_solve_cached) and its
+hand-written adjoint (chz_helmholtz_adjoint / chz_helmholtz_adjoint_batch).
 
     u = helmholtz(f, lam, bc, bcs=(None,) * 6, dom=(0, 1) * 3)
 
@@ -17,12 +17,6 @@ import torch
 import cuhelmholtz_amd as C
 
 
-def _face_of(b, k, batched):
-    if b is None:
-        return None
-    return b[k] if (batched and b.dim() == 2) else b
-
-
 class HelmholtzSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f, lam, bc, dom, *bcs):
@@ -35,9 +29,12 @@ class HelmholtzSolve(torch.autograd.Function):
         if len(lams) != B:
             raise ValueError("lam must be a scalar or hold one value per problem")
         u = fb.clone()
-        for k in range(B):
-            faces = [None if b is None else _face_of(b.detach(), k, batched).contiguous() for b in bcs]
-            C.helmholtz_solve(u[k], (NX, NY, NZ), bc, lams[k], faces, dom)
+        if batched:
+            # one batched solve (persistent launches, the library's cached plan for this geometry)
+            C.helmholtz_solve_batch(u, (NX, NY, NZ), bc, lams, [None if b is None else b.detach() for b in bcs], dom)
+        else:
+            faces = [None if b is None else b.detach().contiguous() for b in bcs]
+            C.helmholtz_solve(u[0], (NX, NY, NZ), bc, lams[0], faces, dom)
         ctx.geom = ((NX, NY, NZ), tuple(bc), tuple(dom), lams, batched, lam_d.dim())
         ctx.face_shapes = [None if b is None else tuple(b.shape) for b in bcs]
         ctx.save_for_backward(u)
@@ -54,18 +51,24 @@ class HelmholtzSolve(torch.autograd.Function):
         glam = torch.zeros(B, dtype=torch.float64, device=u.device)
         gfaces = [None if s is None else torch.zeros(s, dtype=torch.float64, device=u.device)
                   for s in ctx.face_shapes]
-        for k in range(B):
-            g, fg, gl = C.helmholtz_adjoint(u[k], gub[k], n, bc, lams[k], dom, faces=want_faces,
-                                            want_lambda=ctx.needs_input_grad[1])
-            gf[k] = g
+        if batched:
+            gf, fg, gl = C.helmholtz_adjoint_batch(u, gub, n, bc, lams, dom, faces=want_faces,
+                                                   want_lambda=ctx.needs_input_grad[1])
             if gl is not None:
-                glam[k] = gl
+                glam = gl
             for i, s in enumerate(ctx.face_shapes):
                 if s is None or fg[i] is None:
                     continue
-                if batched and len(s) == 2:
-                    gfaces[i][k] = fg[i]
-                else:
+                # per-problem face tensors take their rows, a shared one the sum over the batch
+                gfaces[i] = fg[i].reshape(s) if len(s) == 2 else fg[i].sum(0).reshape(s)
+        else:
+            g, fg, gl = C.helmholtz_adjoint(u[0], gub[0], n, bc, lams[0], dom, faces=want_faces,
+                                            want_lambda=ctx.needs_input_grad[1])
+            gf[0] = g
+            if gl is not None:
+                glam[0] = gl
+            for i, s in enumerate(ctx.face_shapes):
+                if s is not None and fg[i] is not None:
                     gfaces[i] += fg[i].reshape(s)
         g_f = gf if batched else gf[0]
         g_lam = None

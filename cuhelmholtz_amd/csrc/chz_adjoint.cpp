@@ -21,6 +21,7 @@
 
 #include "chz_context.h"
 #include "chz_solver.h"
+#include "../../include/cuhelmholtz.h"
 
 namespace chz {
 
@@ -108,7 +109,115 @@ static int adjoint(const double dom[6], const int N[3], const int bc[3], double 
     return OK;
 }
 
+// Adjoint of `count` problems of one geometry with per-problem lambdas:
+// consecutive node arrays u, gu, gf, per-problem face gradient arrays and
+// lambda gradients.  The node-array steps run once over the whole batch
+// (blockIdx.y = problem); the solve is the batched solve of the library's
+// plan for this geometry (one graph replay, one persistent launch per pass).
+static int adjoint_batch(const double dom[6], const int N[3], const int bc[3], int count, const double *lambdas,
+                         const double *u, const double *gu, double *gf, double *const gfaces[6], double *glambda,
+                         hipStream_t st)
+{
+    if (count <= 0 || !lambdas) return fail(ERR_INVALID, "adjoint batch: empty batch or null lambdas");
+    if (!gu || !gf) return fail(ERR_INVALID, "null gradient array");
+    if (gu == gf) return fail(ERR_INVALID, "the output gradient must not alias the input gradient");
+    if (glambda && !u) return fail(ERR_INVALID, "the lambda gradient needs the forward solution u");
+    const long long fx = N[0] + 1, fy = N[1] + 1, fz = N[2] + 1;
+    const long long nodes = fx * fy * fz;
+    const long long fsz[3] = {fy * fz, fx * fz, fx * fy};
+    const long long maxface = std::max(fsz[0], std::max(fsz[1], fsz[2]));
+    const int nblk = 1024;
+    Context &ctx = Context::get();
+    double *zero, *gw, *part;
+    SolveGeom g;
+    {
+        std::lock_guard<std::mutex> lock(ctx.mutex());
+        zero = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)maxface, 5));
+        gw = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)(nodes * count), 4));
+        part = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)nblk * count, 6));
+        if (!zero || !gw || !part) return fail(ERR_ALLOC, "workspace allocation failed");
+        const double *faces[6] = {zero, zero, zero, zero, zero, zero};
+        int rc = make_geom(g, dom, N, bc, lambdas[0], faces);
+        if (rc) return rc;
+    }
+    AdjArgs a{};
+    for (int d = 0; d < 3; d++) {
+        a.N[d] = g.N[d];
+        a.bc[d] = g.bc[d];
+        a.s[d] = g.s[d];
+        a.e[d] = g.e[d];
+        a.ih[d] = g.ih[d];
+        a.ih2[d] = g.ih2[d];
+    }
+    a.bstride = nodes;
+    for (int f = 0; f < 6; f++) {
+        const int d = f / 2;
+        const bool used = f % 2 == 0 ? (bc[d] == 3 || bc[d] == 4) : (bc[d] == 2 || bc[d] == 3);
+        a.face[f] = used && gfaces ? gfaces[f] : nullptr;
+        a.fstride[f] = fsz[d];
+    }
+    hipError_t e;
+    {
+        double *z = zero;
+        long long n = maxface;
+        void *args[] = {&z, &n};
+        if ((e = hipLaunchKernel(adj_kernel(3), dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)), dim3(256), args, 0, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("adjoint zero fill: ") + hipGetErrorString(e));
+    }
+    {
+        const double *cgu = gu;
+        double *cgf = gf, *cgw = gw;
+        void *args[] = {&a, &cgu, &cgf, &cgw};
+        if ((e = hipLaunchKernel(adj_kernel(0), dim3((unsigned)std::min<long long>((nodes + 255) / 256, 4096), (unsigned)count),
+                                 dim3(256), args, 0, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("adjoint prep launch: ") + hipGetErrorString(e));
+    }
+    // the homogeneous solves of the batch on g (in gw)
+    std::vector<chz_problem> probs((size_t)count);
+    for (int k = 0; k < count; k++) {
+        chz_problem &q = probs[(size_t)k];
+        for (int i = 0; i < 6; i++) q.domain[i] = dom[i];
+        for (int d = 0; d < 3; d++) {
+            q.N[d] = N[d];
+            q.bc[d] = bc[d];
+        }
+        q.lambda = lambdas[k];
+        q.f = gw + nodes * k;
+        for (int i = 0; i < 6; i++) q.faces[i] = g.faces[i] ? zero : nullptr;
+    }
+    int rc = chz_batch_solve_cached(probs.data(), count, 64, 0, st);
+    if (rc) return rc;
+    {
+        const double *cgw = gw, *cu = u;
+        double *cgf = gf, *cp = part;
+        void *args[] = {&a, &cgw, &cu, &cgf, &cp};
+        if ((e = hipLaunchKernel(adj_kernel(1), dim3(nblk, (unsigned)count), dim3(256), args, 0, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("adjoint finish launch: ") + hipGetErrorString(e));
+    }
+    if (glambda) {
+        const double *cp = part;
+        int n = nblk;
+        double *out = glambda;
+        void *args[] = {&cp, &n, &out};
+        if ((e = hipLaunchKernel(adj_kernel(2), dim3((unsigned)count), dim3(256), args, 0, st)) != hipSuccess)
+            return fail(ERR_DEVICE, std::string("adjoint lambda launch: ") + hipGetErrorString(e));
+    }
+    return OK;
+}
+
 }  // namespace chz
+
+extern "C" int chz_helmholtz_adjoint_batch(double left, double right, int NX, int xbc, double bottom, double top, int NY,
+                                           int ybc, double front, double back, int NZ, int zbc, int count,
+                                           const double *lambdas, const double *u, const double *gu, double *gf,
+                                           double *gbcl, double *gbcr, double *gbcb, double *gbct, double *gbcf,
+                                           double *gbce, double *glambda, void *stream)
+{
+    const double dom[6] = {left, right, bottom, top, front, back};
+    const int N[3] = {NX, NY, NZ}, bc[3] = {xbc, ybc, zbc};
+    double *const faces[6] = {gbcl, gbcr, gbcb, gbct, gbcf, gbce};
+    return chz::adjoint_batch(dom, N, bc, count, lambdas, u, gu, gf, faces, glambda, (hipStream_t)stream);
+}
 
 extern "C" int chz_helmholtz_adjoint(double left, double right, int NX, int xbc, double bottom, double top, int NY,
                                      int ybc, double front, double back, int NZ, int zbc, double lambda,

@@ -37,11 +37,42 @@ struct BatchLaunch {
     std::string name;
 };
 
+// geometry of one problem of a plan (a rebind must present the same)
+struct ProbKey {
+    int N[3], bc[3];
+    double dom[6];
+    unsigned faces;   // bit i: face array i read by the codes
+    bool singular;
+    bool operator==(const ProbKey &o) const
+    {
+        for (int a = 0; a < 3; a++)
+            if (N[a] != o.N[a] || bc[a] != o.bc[a]) return false;
+        for (int i = 0; i < 6; i++)
+            if (dom[i] != o.dom[i]) return false;
+        return faces == o.faces && singular == o.singular;
+    }
+};
+
+typedef std::tuple<int, int, int, int, unsigned long long> LaunchKey;   // N, ept, TPL, L, plan
+
+// step s of the problems sharing one launch configuration: one persistent launch
+struct GroupRec {
+    int step;
+    LaunchKey key;
+    std::vector<int> probs;
+    size_t d_off;   // descriptors in the table image
+};
+
 struct BatchPlan {
     int dev = 0, count = 0, prec = 64, refine = 0;
     void *work = nullptr;     // per-problem workspaces
     void *tables = nullptr;   // descriptors, items, periodic copies
     size_t work_bytes = 0, table_bytes = 0;
+    std::vector<char> host;               // host image of the tables
+    std::vector<GroupRec> groups;
+    size_t c_off[3] = {0, 0, 0};          // periodic copy lists in the image
+    std::vector<size_t> ws_off, u_off;    // workspace / iterate offsets in work
+    std::vector<ProbKey> keys;
     std::vector<BatchLaunch> launches;
     const BatchCopy *copies[3] = {nullptr, nullptr, nullptr};
     int ncopies[3] = {0, 0, 0}, copy_grid[3] = {0, 0, 0};
@@ -95,6 +126,65 @@ static int enqueue(Context &ctx, const BatchPlan &p, hipStream_t st, bool timed)
     return OK;
 }
 
+static ProbKey key_of(const SolveGeom &g, const chz_problem &q)
+{
+    ProbKey k{};
+    for (int a = 0; a < 3; a++) {
+        k.N[a] = g.N[a];
+        k.bc[a] = g.bc[a];
+    }
+    for (int i = 0; i < 6; i++) {
+        k.dom[i] = q.domain[i];
+        if (g.faces[i]) k.faces |= 1u << i;
+    }
+    k.singular = g.singular;
+    return k;
+}
+
+// pass sequence of every problem (the single solve's builder) on the plan's workspaces
+static int build_specs(Context &ctx, const chz_problem *probs, const BatchPlan &plan, const std::vector<SolveGeom> &geo,
+                       std::vector<std::vector<PassSpec>> &specs)
+{
+    char *wbase = reinterpret_cast<char *>(plan.work);
+    const int refine = plan.refine;
+    specs.assign(plan.count, {});
+    for (int i = 0; i < plan.count; i++) {
+        const SolveGeom &g = geo[i];
+        double *f = probs[i].f;
+        std::vector<PassSpec> &v = specs[i];
+        int rc;
+        if (plan.prec == 64) {
+            double *ex = reinterpret_cast<double *>(wbase + plan.ws_off[i]);
+            if (!(rc = solve_passes<double>(ctx, g, f, ex, nullptr, nullptr, v))) rc = final_pass<double>(ctx, g, ex, nullptr, f, v);
+        } else {
+            float *ex = reinterpret_cast<float *>(wbase + plan.ws_off[i]);
+            double *U = refine > 0 ? reinterpret_cast<double *>(wbase + plan.u_off[i]) : nullptr;
+            if (!(rc = solve_passes<float>(ctx, g, f, ex, nullptr, nullptr, v)))
+                rc = final_pass<float>(ctx, g, ex, nullptr, refine > 0 ? U : f, v);
+            for (int it = 1; it <= refine && rc == OK; it++)
+                if (!(rc = solve_passes<float>(ctx, g, f, ex, U, nullptr, v))) rc = final_pass<float>(ctx, g, ex, U, it == refine ? f : U, v);
+        }
+        if (rc) return fail(rc, "problem " + std::to_string(i) + ": " + last_error());
+    }
+    return OK;
+}
+
+// descriptors of every group and the periodic copy lists into the table image
+static void write_descs(BatchPlan &plan, const chz_problem *probs, const std::vector<SolveGeom> &geo,
+                        const std::vector<std::vector<PassSpec>> &specs)
+{
+    for (const GroupRec &gr : plan.groups) {
+        BatchDesc *d = reinterpret_cast<BatchDesc *>(plan.host.data() + gr.d_off);
+        for (size_t k = 0; k < gr.probs.size(); k++) d[k] = specs[gr.probs[k]][gr.step].d;
+    }
+    for (int a = 0; a < 3; a++) {
+        BatchCopy *c = reinterpret_cast<BatchCopy *>(plan.host.data() + plan.c_off[a]);
+        int n = 0;
+        for (int i = 0; i < plan.count; i++)
+            if (geo[i].bc[a] == 0) c[n++] = BatchCopy{probs[i].f, geo[i].N[0], geo[i].N[1], geo[i].N[2], a};
+    }
+}
+
 static int create(const chz_problem *probs, int count, int prec, int refine, BatchPlan **out)
 {
     if (!out) return fail(ERR_INVALID, "null plan pointer");
@@ -108,7 +198,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
 
     std::vector<SolveGeom> geo(count);
     const size_t esz = prec == 64 ? 8 : 4;
-    std::vector<size_t> ws_off(count), u_off(count);
+    std::vector<size_t> ws_off(count), u_off(count, 0);
     size_t total = 0;
     for (int i = 0; i < count; i++) {
         const chz_problem &q = probs[i];
@@ -132,32 +222,17 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     plan->refine = refine;
     plan->dev = ctx.device();
     plan->work_bytes = total;
+    plan->ws_off = ws_off;
+    plan->u_off = u_off;
+    for (int i = 0; i < count; i++) plan->keys.push_back(key_of(geo[i], probs[i]));
     if (hipMalloc(&plan->work, total) != hipSuccess) {
         delete plan;
         return fail(ERR_ALLOC, "batched workspace allocation failed (" + std::to_string(total) + " bytes)");
     }
-    char *wbase = reinterpret_cast<char *>(plan->work);
 
     // pass sequence of every problem
-    std::vector<std::vector<PassSpec>> specs(count);
-    int rc = OK;
-    for (int i = 0; i < count && rc == OK; i++) {
-        const SolveGeom &g = geo[i];
-        double *f = probs[i].f;
-        std::vector<PassSpec> &v = specs[i];
-        if (prec == 64) {
-            double *ex = reinterpret_cast<double *>(wbase + ws_off[i]);
-            if (!(rc = solve_passes<double>(ctx, g, f, ex, nullptr, nullptr, v))) rc = final_pass<double>(ctx, g, ex, nullptr, f, v);
-        } else {
-            float *ex = reinterpret_cast<float *>(wbase + ws_off[i]);
-            double *U = refine > 0 ? reinterpret_cast<double *>(wbase + u_off[i]) : nullptr;
-            if (!(rc = solve_passes<float>(ctx, g, f, ex, nullptr, nullptr, v)))
-                rc = final_pass<float>(ctx, g, ex, nullptr, refine > 0 ? U : f, v);
-            for (int it = 1; it <= refine && rc == OK; it++)
-                if (!(rc = solve_passes<float>(ctx, g, f, ex, U, nullptr, v))) rc = final_pass<float>(ctx, g, ex, U, it == refine ? f : U, v);
-        }
-        if (rc) rc = fail(rc, "problem " + std::to_string(i) + ": " + last_error());
-    }
+    std::vector<std::vector<PassSpec>> specs;
+    int rc = build_specs(ctx, probs, *plan, geo, specs);
     if (rc) {
         delete plan;
         return rc;
@@ -172,32 +247,27 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     // group step s of all problems by launch configuration
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->dev) != hipSuccess || cus <= 0) cus = 256;
-    struct Group {
-        int step;
-        const PassSpec *first;
-        std::vector<int> probs;
-    };
-    std::vector<Group> groups;
+    std::vector<GroupRec> &groups = plan->groups;
     for (size_t s = 0; s < nsteps; s++) {
-        std::map<std::tuple<int, int, int, int, unsigned long long>, size_t> idx;
+        std::map<LaunchKey, size_t> idx;
         for (int i = 0; i < count; i++) {
             const PassSpec &q = specs[i][s];
-            auto key = std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+            const LaunchKey key = std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
             auto it = idx.find(key);
             if (it == idx.end()) {
                 idx[key] = groups.size();
-                groups.push_back(Group{(int)s, &q, {i}});
+                groups.push_back(GroupRec{(int)s, key, {i}, 0});
             } else {
                 groups[it->second].probs.push_back(i);
             }
         }
     }
     // table layout: per group descriptors then items
-    std::vector<size_t> d_off(groups.size()), i_off(groups.size());
+    std::vector<size_t> i_off(groups.size());
     std::vector<int> nitems(groups.size());
     size_t tb = 0;
     for (size_t gi = 0; gi < groups.size(); gi++) {
-        const Group &gr = groups[gi];
+        GroupRec &gr = groups[gi];
         long long n = 0;
         for (int i : gr.probs) {
             const PassSpec &q = specs[i][gr.step];
@@ -208,41 +278,39 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
             return fail(ERR_INVALID, "batched pass with more than 2^30 tiles");
         }
         nitems[gi] = (int)n;
-        d_off[gi] = tb;
+        gr.d_off = tb;
         tb += align256(sizeof(BatchDesc) * gr.probs.size());
         i_off[gi] = tb;
         tb += align256(sizeof(BatchItem) * (size_t)n);
     }
-    size_t c_off[3];
-    std::vector<BatchCopy> copies[3];
     for (int a = 0; a < 3; a++) {
-        int maxface = 0;
+        int maxface = 0, n = 0;
         for (int i = 0; i < count; i++) {
             const SolveGeom &g = geo[i];
             if (g.bc[a] != 0) continue;
-            copies[a].push_back(BatchCopy{probs[i].f, g.N[0], g.N[1], g.N[2], a});
+            n++;
             const int face = a == 0 ? (g.N[1] + 1) * (g.N[2] + 1) : a == 1 ? (g.N[0] + 1) * (g.N[2] + 1)
                                                                              : (g.N[0] + 1) * (g.N[1] + 1);
             maxface = std::max(maxface, face);
         }
-        if (copies[a].size() > 65535) {
+        if (n > 65535) {
             delete plan;
             return fail(ERR_INVALID, "more than 65535 periodic problems along one axis");
         }
-        plan->ncopies[a] = (int)copies[a].size();
+        plan->ncopies[a] = n;
         plan->copy_grid[a] = std::min(64, (maxface + 255) / 256);
-        c_off[a] = tb;
-        tb += align256(sizeof(BatchCopy) * copies[a].size());
+        plan->c_off[a] = tb;
+        tb += align256(sizeof(BatchCopy) * (size_t)n);
     }
-    std::vector<char> host(tb, 0);
+    std::vector<char> &host = plan->host;
+    host.assign(tb, 0);
+    write_descs(*plan, probs, geo, specs);
     for (size_t gi = 0; gi < groups.size(); gi++) {
-        const Group &gr = groups[gi];
-        BatchDesc *d = reinterpret_cast<BatchDesc *>(host.data() + d_off[gi]);
+        const GroupRec &gr = groups[gi];
         BatchItem *it = reinterpret_cast<BatchItem *>(host.data() + i_off[gi]);
         int w = 0;
         for (size_t k = 0; k < gr.probs.size(); k++) {
             const PassSpec &q = specs[gr.probs[k]][gr.step];
-            d[k] = q.d;
             // tile t = a + ntiles * b; listed a-fastest, or b-fastest where the
             // pass reads neighbour planes (the residual: consecutive items then
             // share rows of U in L2)
@@ -255,8 +323,6 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
             }
         }
     }
-    for (int a = 0; a < 3; a++)
-        if (!copies[a].empty()) memcpy(host.data() + c_off[a], copies[a].data(), sizeof(BatchCopy) * copies[a].size());
     plan->table_bytes = tb;
     if (hipMalloc(&plan->tables, tb) != hipSuccess) {
         delete plan;
@@ -267,11 +333,11 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         return fail(ERR_DEVICE, "batched descriptor table upload failed");
     }
     char *tbase = reinterpret_cast<char *>(plan->tables);
-    for (int a = 0; a < 3; a++) plan->copies[a] = reinterpret_cast<const BatchCopy *>(tbase + c_off[a]);
+    for (int a = 0; a < 3; a++) plan->copies[a] = reinterpret_cast<const BatchCopy *>(tbase + plan->c_off[a]);
 
     for (size_t gi = 0; gi < groups.size(); gi++) {
-        const Group &gr = groups[gi];
-        const PassSpec &q = *gr.first;
+        const GroupRec &gr = groups[gi];
+        const PassSpec &q = specs[gr.probs[0]][gr.step];
         BatchLaunch l;
         l.lds = axis_pass_lds(q.prec, q.d.p.M, q.d.p.L, q.d.p.TPL);
         // lean tile I/O where the windows of every problem of the group allow it
@@ -297,7 +363,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         l.grid = (int)std::max<long long>(grid, 1);
         l.items = reinterpret_cast<const BatchItem *>(tbase + i_off[gi]);
         l.nitems = nitems[gi];
-        l.descs = reinterpret_cast<const BatchDesc *>(tbase + d_off[gi]);
+        l.descs = reinterpret_cast<const BatchDesc *>(tbase + gr.d_off);
         l.name = std::string(q.name) + "/" + std::to_string(q.N);
         if (l.nitems > 0) plan->launches.push_back(l);
     }
@@ -326,6 +392,120 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         return fail(ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
     *out = plan;
+    return OK;
+}
+
+// Point a plan at a new problem list of the same geometry (sizes, codes,
+// domains, which face arrays, singularity): new f / face arrays and lambdas.
+// The descriptors and periodic copy lists are rebuilt and uploaded on st after
+// the work already queued there (the items, workspaces and graph stay).
+static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStream_t st)
+{
+    if (!probs || count != plan.count) return fail(ERR_INVALID, "rebind: the problem count differs from the plan's");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != plan.dev) return fail(ERR_INVALID, "plan belongs to another device");
+    Context &ctx = Context::get();
+    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::vector<SolveGeom> geo(count);
+    for (int i = 0; i < count; i++) {
+        const chz_problem &q = probs[i];
+        if (!q.f) return fail(ERR_INVALID, "problem " + std::to_string(i) + ": null solution array");
+        int rc = make_geom(geo[i], q.domain, q.N, q.bc, q.lambda, q.faces);
+        if (rc) return fail(rc, "problem " + std::to_string(i) + ": " + last_error());
+        if (!(key_of(geo[i], q) == plan.keys[i]))
+            return fail(ERR_INVALID, "rebind: problem " + std::to_string(i) + " differs in geometry from the plan's");
+    }
+    std::vector<std::vector<PassSpec>> specs;
+    int rc = build_specs(ctx, probs, plan, geo, specs);
+    if (rc) return rc;
+    for (const GroupRec &gr : plan.groups)
+        for (int i : gr.probs) {
+            const PassSpec &q = specs[i][gr.step];
+            if (std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan) != gr.key)
+                return fail(ERR_INVALID, "rebind: launch configurations changed since the plan was made");
+        }
+    // the image is rewritten only after the stream's earlier uploads have run
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+        write_descs(plan, probs, geo, specs);
+        e = hipMemcpyAsync(plan.tables, plan.host.data(), plan.table_bytes, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("rebind upload: ") + hipGetErrorString(e));
+    return OK;
+}
+
+// Plans kept for repeated solves of one geometry (chz_batch_solve_cached):
+// up to kCached per process, least recently used first out.  Never freed at
+// exit (the device may be gone by then).
+static const size_t kCached = 4;
+struct PlanCache {
+    std::mutex mu;
+    std::vector<std::pair<std::string, BatchPlan *>> plans;   // most recent last
+};
+static PlanCache &plan_cache()
+{
+    static PlanCache *c = new PlanCache();
+    return *c;
+}
+
+static std::string signature(const chz_problem *probs, int count, int prec, int refine, int dev)
+{
+    std::string s;
+    auto put = [&s](const void *p, size_t n) { s.append(reinterpret_cast<const char *>(p), n); };
+    put(&dev, sizeof dev);
+    put(&prec, sizeof prec);
+    put(&refine, sizeof refine);
+    put(&count, sizeof count);
+    for (int i = 0; i < count; i++) {
+        const chz_problem &q = probs[i];
+        put(q.N, sizeof q.N);
+        put(q.bc, sizeof q.bc);
+        put(q.domain, sizeof q.domain);
+        unsigned faces = 0;
+        for (int k = 0; k < 6; k++)
+            if (q.faces[k]) faces |= 1u << k;
+        put(&faces, sizeof faces);
+        const char zero = q.lambda == 0.0;   // singularity also depends on lambda = 0
+        put(&zero, 1);
+    }
+    return s;
+}
+
+static int solve_cached(const chz_problem *probs, int count, int prec, int refine, hipStream_t st)
+{
+    if (!probs || count <= 0) return fail(ERR_INVALID, "empty problem list");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(ERR_DEVICE, "no current device");
+    const std::string sig = signature(probs, count, prec, refine, dev);
+    PlanCache &c = plan_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    BatchPlan *plan = nullptr;
+    for (size_t i = 0; i < c.plans.size(); i++)
+        if (c.plans[i].first == sig) {
+            plan = c.plans[i].second;
+            c.plans.erase(c.plans.begin() + (long)i);
+            break;
+        }
+    int rc;
+    if (plan) {
+        if ((rc = rebind(*plan, probs, count, st))) {
+            (void)hipDeviceSynchronize();   // the dropped plan may still run on another stream
+            delete plan;
+            return rc;
+        }
+    } else {
+        if ((rc = create(probs, count, prec, refine, &plan))) return rc;
+        if (c.plans.size() >= kCached) {
+            // the evicted plan may still run on some stream
+            (void)hipDeviceSynchronize();
+            delete c.plans.front().second;
+            c.plans.erase(c.plans.begin());
+        }
+    }
+    c.plans.push_back({sig, plan});
+    hipError_t e = hipGraphLaunch(plan->exec, st);
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
     return OK;
 }
 
@@ -361,6 +541,17 @@ int chz_batch_solve(void *plan, int use_graph, void *stream)
     int rc = chz::enqueue(ctx, *p, st, tm);
     if (tm) ctx.timing_end(st);
     return rc;
+}
+
+int chz_batch_rebind(void *plan, const chz_problem *problems, int count, void *stream)
+{
+    if (!plan) return chz::fail(chz::ERR_INVALID, "null plan");
+    return chz::rebind(*reinterpret_cast<BatchPlan *>(plan), problems, count, (hipStream_t)stream);
+}
+
+int chz_batch_solve_cached(const chz_problem *problems, int count, int precision, int refine, void *stream)
+{
+    return chz::solve_cached(problems, count, precision, refine, (hipStream_t)stream);
 }
 
 int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *table_bytes)

@@ -19,9 +19,16 @@ __device__ __forceinline__ double axis_weight(int bc, int q, int N)
 // output gradient on every node that is not a periodic duplicate (0 on the
 // duplicates); g = gf / W on the unknown box, 0 elsewhere (the rhs of the
 // adjoint solve, homogeneous boundary data).  Zeroes the face gradients.
+// Problem blockIdx.y of a batch.
 __global__ void __launch_bounds__(256) k_adj_prep(AdjArgs a, const double *__restrict__ gu, double *__restrict__ gf,
                                                   double *__restrict__ g)
 {
+    const long long y = blockIdx.y;
+    gu += y * a.bstride;
+    gf += y * a.bstride;
+    g += y * a.bstride;
+    for (int f = 0; f < 6; f++)
+        if (a.face[f]) a.face[f] += y * a.fstride[f];
     const long long fx = a.N[0] + 1, fy = a.N[1] + 1, fz = a.N[2] + 1;
     const long long total = fx * fy * fz;
     const long long fsz[3] = {fy * fz, fx * fz, fx * fy};
@@ -66,10 +73,18 @@ __global__ void __launch_bounds__(256) k_adj_prep(AdjArgs a, const double *__res
 // r = W g on the unknown box (the adjoint solution): gf = r there; the fold
 // transposes (Dirichlet boundary node -= r / h^2, Neumann face gradient
 // +-2 r / h); part[block] = sum of r u (u = forward solution, may be null).
+// Problem blockIdx.y of a batch (its partial sums at part + blockIdx.y * gridDim.x).
 __global__ void __launch_bounds__(256) k_adj_finish(AdjArgs a, const double *__restrict__ g, const double *__restrict__ u,
                                                     double *__restrict__ gf, double *__restrict__ part)
 {
     __shared__ double red[256];
+    const long long y = blockIdx.y;
+    g += y * a.bstride;
+    if (u) u += y * a.bstride;
+    gf += y * a.bstride;
+    part += y * gridDim.x;
+    for (int f = 0; f < 6; f++)
+        if (a.face[f]) a.face[f] += y * a.fstride[f];
     const long long fx = a.N[0] + 1, fy = a.N[1] + 1;
     const long long nx = a.e[0] - a.s[0] + 1, ny = a.e[1] - a.s[1] + 1, nz = a.e[2] - a.s[2] + 1;
     const long long total = nx * ny * nz;
@@ -110,10 +125,12 @@ __global__ void __launch_bounds__(256) k_adj_finish(AdjArgs a, const double *__r
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// out[0] = -sum of part[0..n) in a fixed order (one block)
+// out[b] = -sum of part[b*n .. b*n + n) in a fixed order (block b: problem b of a batch)
 __global__ void __launch_bounds__(256) k_adj_lambda(const double *__restrict__ part, int n, double *__restrict__ out)
 {
     __shared__ double red[256];
+    part += (long long)blockIdx.x * n;
+    out += blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
     red[threadIdx.x] = s;

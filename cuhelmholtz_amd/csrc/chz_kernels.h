@@ -97,6 +97,9 @@ struct AdjArgs {
     int N[3], bc[3], s[3], e[3];
     double ih[3], ih2[3];
     double *face[6];   // gradient face arrays (null where not wanted)
+    // batch of problems of this geometry (blockIdx.y = problem): node-array
+    // and face-array strides between consecutive problems (0: one problem)
+    long long bstride, fstride[6];
 };
 // adjoint kernels: 0 prep, 1 finish, 2 lambda reduction, 3 zero fill
 const void *adj_kernel(int which);

@@ -104,6 +104,19 @@ int chz_helmholtz_adjoint(double left, double right, int NX, int xbc,
                           double *gbcl, double *gbcr, double *gbcb, double *gbct, double *gbcf, double *gbce,
                           double *glambda, void *stream);
 
+/* chz_helmholtz_adjoint for `count` problems of one geometry and codes with
+ * lambdas[k] (host array): u, gu, gf hold `count` consecutive node arrays,
+ * each face gradient (NULL = not wanted) `count` consecutive face arrays, and
+ * glambda (NULL = not wanted) `count` values.  The node-array steps run once
+ * over the batch and the solves as one batched solve (chz_batch_solve_cached
+ * plan).  No reference counterpart. */
+int chz_helmholtz_adjoint_batch(double left, double right, int NX, int xbc,
+                                double bottom, double top, int NY, int ybc,
+                                double front, double back, int NZ, int zbc,
+                                int count, const double *lambdas, const double *u, const double *gu, double *gf,
+                                double *gbcl, double *gbcr, double *gbcb, double *gbct, double *gbcf, double *gbce,
+                                double *glambda, void *stream);
+
 /* One problem of a batched solve: the arguments of chz_helmholtz_solve. */
 typedef struct chz_problem {
     double domain[6];        /* left, right, bottom, top, front, back */
@@ -131,6 +144,22 @@ int chz_batch_create(const chz_problem *problems, int count, int precision, int 
  * graph (one hipGraphLaunch); 0 issues the same launches directly (and
  * records per-launch timing when chz_set_timing is on). */
 int chz_batch_solve(void *plan, int use_graph, void *stream);
+
+/* Point a plan at `count` problems of the same geometry as the ones it was
+ * made for (sizes, codes, domains, which face arrays are given, lambda = 0 or
+ * not): new f and face arrays and new lambdas.  Waits for the work queued on
+ * `stream`, then rewrites and uploads the pass descriptors on it; the plan's
+ * workspaces, work lists and graph are kept.  Calls that overlap with a
+ * solve of the same plan on another stream race.  No reference counterpart. */
+int chz_batch_rebind(void *plan, const chz_problem *problems, int count, void *stream);
+
+/* Batched solve through a plan kept by the library for repeated solves of
+ * one geometry (a training loop, a time stepper): the first call makes the
+ * plan as chz_batch_create does, later calls with the same geometry rebind it
+ * (chz_batch_rebind) and replay its graph on `stream`.  Up to 4 plans are
+ * kept per process (least recently used dropped first).  No reference
+ * counterpart. */
+int chz_batch_solve_cached(const chz_problem *problems, int count, int precision, int refine, void *stream);
 
 /* Launch count of one batched solve and the plan's device memory. */
 int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *table_bytes);

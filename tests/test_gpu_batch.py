@@ -155,3 +155,96 @@ def test_batch_errors():
                        (p["f"], (8, 8, 8), (7, 3, 3), -1.0, p["bcs"])])
     with pytest.raises(C.CHZError):
         C.BatchSolver([(p["f"], (8, 8, 8), (3, 3, 3), -1.0, p["bcs"])], prec=64, refine=1)
+
+
+def test_batch_rebind_new_arrays_and_lambdas():
+    """a plan rebound to new f / face arrays and lambdas solves those (same as a fresh single solve)"""
+    probs = make(small_descs(12, 21))
+    bs = run_batch(probs)
+    for p in probs:
+        assert rel(p["f"], single(p)) <= SAME
+    # new right-hand sides, new face data, new lambdas (same geometry: lambda keeps its zero / non-zero class)
+    fresh = []
+    for k, p in enumerate(probs):
+        d = dict(p["d"])
+        if d["lam"] != 0.0:
+            d["lam"] = d["lam"] * 0.5 - 0.25
+        f = torch.randn_like(p["f0"])
+        bcs = [None if b is None else torch.randn_like(b) for b in p["bcs"]]
+        fresh.append({"d": d, "f0": f.clone(), "f": f, "bcs": bcs})
+    bs.rebind([(q["f"], q["d"]["n"], q["d"]["bc"], q["d"]["lam"], q["bcs"]) for q in fresh])
+    bs.solve()
+    torch.cuda.synchronize()
+    for k, q in enumerate(fresh):
+        e = rel(q["f"], single(q))
+        assert e <= SAME, (k, q["d"], e)
+    # a different geometry is refused
+    bad = [(q["f"], q["d"]["n"], q["d"]["bc"], q["d"]["lam"], q["bcs"]) for q in fresh]
+    n0 = fresh[0]["d"]["n"]
+    bad[0] = (fresh[1]["f"], fresh[1]["d"]["n"], fresh[1]["d"]["bc"], fresh[1]["d"]["lam"], fresh[1]["bcs"])
+    if fresh[1]["d"]["n"] != n0 or fresh[1]["d"]["bc"] != fresh[0]["d"]["bc"]:
+        with pytest.raises(C.CHZError, match="geometry"):
+            bs.rebind(bad)
+    bs.close()
+
+
+@pytest.mark.parametrize("B,n,bc", [(4, (16, 12, 20), (3, 2, 0)), (16, (32, 32, 32), (1, 3, 0))])
+def test_solve_batch_cached_matches_single(B, n, bc):
+    """helmholtz_solve_batch (cached plan, rebound on the second call) equals single solves"""
+    NX, NY, NZ = n
+    g = torch.Generator(device=dev())
+    sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+    for call in range(3):
+        g.manual_seed(100 + call)
+        f = torch.randn((B, NZ + 1, NY + 1, NX + 1), generator=g, dtype=torch.float64, device=dev())
+        lams = [-1.0 - 0.5 * k - call for k in range(B)]
+        bcs = [None] * 6
+        for i in range(6):
+            if bc[i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)):
+                # face 0 / 2 / 4 shared by the batch, 1 / 3 / 5 one per problem
+                shape = (sizes[i],) if i % 2 == 0 else (B, sizes[i])
+                bcs[i] = torch.randn(shape, generator=g, dtype=torch.float64, device=dev())
+        u = f.clone()
+        C.helmholtz_solve_batch(u, n, bc, lams, bcs)
+        torch.cuda.synchronize()
+        for k in range(B):
+            fk = [None if b is None else (b[k] if b.dim() == 2 else b) for b in bcs]
+            want = f[k].clone()
+            C.helmholtz_solve(want, n, bc, lams[k], fk)
+            assert rel(u[k], want) <= SAME, (call, k, rel(u[k], want))
+
+
+def test_adjoint_batch_matches_single():
+    """the batched adjoint equals the single-problem adjoint for every problem, gradients of faces and lambda included"""
+    n, bc, B = (12, 16, 10), (4, 3, 0), 5
+    NX, NY, NZ = n
+    g = torch.Generator(device=dev())
+    g.manual_seed(31)
+    u = torch.randn((B, NZ + 1, NY + 1, NX + 1), generator=g, dtype=torch.float64, device=dev())
+    gu = torch.randn((B, NZ + 1, NY + 1, NX + 1), generator=g, dtype=torch.float64, device=dev())
+    lams = [-0.5 - k for k in range(B)]
+    gf, gfaces, glam = C.helmholtz_adjoint_batch(u, gu, n, bc, lams, faces=True)
+    torch.cuda.synchronize()
+    for k in range(B):
+        g1, f1, l1 = C.helmholtz_adjoint(u[k].contiguous(), gu[k].contiguous(), n, bc, lams[k], faces=True)
+        assert rel(gf[k], g1) <= SAME, k
+        for i in range(6):
+            assert (gfaces[i] is None) == (f1[i] is None)
+            if f1[i] is not None:
+                assert rel(gfaces[i][k], f1[i]) <= SAME, (k, i)
+        assert abs(float(glam[k]) - float(l1)) <= SAME * max(1.0, abs(float(l1))), k
+
+
+def test_solve_batch_cache_evicts():
+    """more geometries than the plan cache holds: every call still solves its own problems"""
+    for it in range(2):
+        for N in (8, 10, 12, 16, 20, 24):
+            n = (N, N, 8)
+            f = torch.randn((2, 9, N + 1, N + 1), dtype=torch.float64, device=dev())
+            u = f.clone()
+            C.helmholtz_solve_batch(u, n, (1, 1, 0), [-1.0, -2.0])
+            torch.cuda.synchronize()
+            for k in range(2):
+                want = f[k].clone()
+                C.helmholtz_solve(want, n, (1, 1, 0), [-1.0, -2.0][k])
+                assert rel(u[k], want) <= SAME, (it, N, k)
