@@ -449,6 +449,17 @@ static PlanCache &plan_cache()
     return *c;
 }
 
+// drop every cached plan (chz_release)
+void release_cached_plans()
+{
+    PlanCache &c = plan_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (c.plans.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto &e : c.plans) delete e.second;
+    c.plans.clear();
+}
+
 static std::string signature(const chz_problem *probs, int count, int prec, int refine, int dev)
 {
     std::string s;

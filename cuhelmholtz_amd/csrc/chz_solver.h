@@ -45,6 +45,9 @@ int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin,
 int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st);
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 
+// free the batched plans kept by chz_batch_solve_cached (waits for the device)
+void release_cached_plans();
+
 int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,
                     double *f, int refine, hipStream_t st);
 int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const double *const faces[6], double lambda,

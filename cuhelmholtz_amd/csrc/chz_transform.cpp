@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "chz_context.h"
+#include "chz_solver.h"
 #include "chz_transform.h"
 
 namespace chz {
@@ -193,7 +194,11 @@ int chz_do_transform_f32(float *d_data, float *d_out, int NX, int NY, int NZ, in
 
 const char *chz_last_error(void) { return chz::last_error(); }
 
-void chz_release(void) { chz::Context::get().release(); }
+void chz_release(void)
+{
+    chz::release_cached_plans();
+    chz::Context::get().release();
+}
 
 int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines, unsigned long long *plan)
 {

@@ -170,7 +170,8 @@ int chz_batch_destroy(void *plan);
 /* Message of the last failed call on this thread ("" if none). */
 const char *chz_last_error(void);
 
-/* Free cached twiddle tables and workspaces of the current device.
+/* Free cached twiddle tables and workspaces of the current device, and the
+ * batched plans kept by chz_batch_solve_cached (after waiting for the device).
  * Replaces freeMemory_cubic (reference cudasymmfft/funcinterface.cu:337-342). */
 void chz_release(void);
 
