@@ -10,6 +10,8 @@ const FixedEntry *line_table_f64(int *n);
 const FixedEntry *line_table_f32(int *n);
 const FixedEntry *line_small_table_f64(int *n);
 const FixedEntry *line_small_table_f32(int *n);
+const FixedEntry *line_mid_table_f64(int *n);
+const FixedEntry *line_mid_table_f32(int *n);
 const FixedEntry *fixed_table_f32(int *n);
 const FixedEntry *batch_fixed_table_64_0(int *n);
 const FixedEntry *batch_fixed_table_64_1(int *n);
@@ -40,9 +42,11 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
     static const bool no_line = getenv("CHZ_NO_LINE") != nullptr;
     int n = 0;
     if (!no_line && p && line_windows_ok(*p, N)) {
-        for (int q = 0; q < 2; q++) {
-            const FixedEntry *t = prec == 64 ? (q ? line_small_table_f64(&n) : line_table_f64(&n))
-                                             : (q ? line_small_table_f32(&n) : line_table_f32(&n));
+        typedef const FixedEntry *(*Tab)(int *);
+        static const Tab t64[] = {line_table_f64, line_small_table_f64, line_mid_table_f64};
+        static const Tab t32[] = {line_table_f32, line_small_table_f32, line_mid_table_f32};
+        for (int q = 0; q < 3; q++) {
+            const FixedEntry *t = (prec == 64 ? t64 : t32)[q](&n);
             for (int i = 0; i < n; i++) {
                 const FixedEntry &e = t[i];
                 if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;

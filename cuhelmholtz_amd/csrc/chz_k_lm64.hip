@@ -1,0 +1,34 @@
+// FP64 fixed-shape pass kernels with lean tile I/O for N = 256, 384
+// (chz_line.h, k_line): N = 256 and 384 (the 256^3 and 512x384x256 workloads),
+// the shapes tuning/gfx950.txt selects and the alternatives measured against them.
+#include "chz_line.h"
+namespace chz {
+
+#define LM(T, PREC, OP, N, EPT, TPL, L, PLAN)                                                        \
+    {PREC, OP, N, EPT, TPL, L, PLAN, (const void *)k_line<T, OP, N, TPL, L, PLAN, EPT>,              \
+     line_lds_extra<T, N, OP>()}
+
+static const FixedEntry kTab[] = {
+    LM(double, 64, B_ROWS, 256, 8, 32, 8, 0x20808ull),
+    LM(double, 64, B_ROWS, 256, 8, 16, 16, 0x20808ull),
+    LM(double, 64, B_XFWD, 256, 8, 32, 8, 0x20808ull),
+    LM(double, 64, B_XFWD, 256, 8, 16, 16, 0x20808ull),
+    LM(double, 64, B_COLS, 256, 8, 16, 16, 0x20808ull),
+    LM(double, 64, B_COLS, 256, 8, 32, 16, 0x20808ull),
+    LM(double, 64, B_ZMID, 256, 8, 16, 16, 0x20808ull),
+    LM(double, 64, B_ZMID, 256, 8, 32, 16, 0x20808ull),
+    LM(double, 64, B_ROWS, 384, 8, 32, 8, 0x30808ull),
+    LM(double, 64, B_XFWD, 384, 8, 32, 8, 0x30808ull),
+    LM(double, 64, B_COLS, 384, 8, 32, 8, 0x30808ull),
+    LM(double, 64, B_COLS, 384, 8, 32, 16, 0x30808ull),
+    LM(double, 64, B_ZMID, 384, 8, 32, 8, 0x30808ull),
+    LM(double, 64, B_ZMID, 384, 8, 32, 16, 0x30808ull)
+};
+
+const FixedEntry *line_mid_table_f64(int *n)
+{
+    *n = (int)(sizeof(kTab) / sizeof(kTab[0]));
+    return kTab;
+}
+
+}  // namespace chz
