@@ -151,7 +151,7 @@ def test_vs_hw3crt_medium(n, bc):
     assert ierr == 0 and rel(got, v) < TOL
 
 
-@pytest.mark.parametrize("n", [(512, 512, 512), (512, 384, 256)])
+@pytest.mark.parametrize("n", [(512, 512, 512), (512, 384, 256), (256, 256, 256), (384, 384, 384)])
 @pytest.mark.parametrize("lam", [-4.0, 0.5])
 def test_full_size_residual(n, lam):
     """Mixed-precision workload shape, plain FP64 path: residual invariant."""
@@ -234,7 +234,7 @@ def test_mixed_singular_and_periodic():
         assert rel(mixed_solve(f, n, bc, 0.0, bcs, 2), ref) < 1e-12, bc
 
 
-@pytest.mark.parametrize("n", [(512, 384, 256), (512, 512, 512)])
+@pytest.mark.parametrize("n", [(512, 384, 256), (512, 512, 512), (256, 256, 256), (384, 384, 384)])
 @pytest.mark.parametrize("lam", [-4.0, 0.5])
 def test_mixed_full_size_1e10(n, lam):
     """Mixed-precision workload: FP32 + refinement reaches 1e-10 relative error
