@@ -30,7 +30,10 @@ Prints ONE JSON line (rank 0):
   config.fp64  also: per-step median, the first call's time (workspace and
               table setup included) and an end-to-end solve from host arrays
               through hw3crt_ (allocation and PCIe copies included)
+  config.other_sizes  the 3D transform at 128^3, 256^3, 512x384x256 and the
+              FP64 / FP32+refinement solves at 512x384x256
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-adjoint] [--no-e2e]
+                       [--no-sizes]
 """
 import argparse
 import json
@@ -194,6 +197,68 @@ def bench_batched(dev, steps, count=2048):
     return out
 
 
+def _median_ms(fn, reps):
+    import numpy as np
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def bench_sizes(dev, reps=10):
+    """The other sizes of the workloads: the 3D transform (DCT-I, DCT-III,
+    R2C) at 128^3, 256^3 and 512x384x256, and the solves of the mixed-precision
+    workload at 512x384x256 (codes (3,4,0), lambda = -4): FP64 and FP32 with
+    the fewest refinement steps reaching 1e-10 against FP64.  Medians of single
+    calls."""
+    import torch
+    import cuhelmholtz_amd as C
+    from cuhelmholtz_amd import workloads as W
+    out = {"transform_ms": {}}
+    kinds = (C.DCT_1, C.DCT_3, C.DFT_R2C)
+    for n in ((128, 128, 128), (256, 256, 256), (512, 384, 256)):
+        x = torch.zeros((n[2] + 2, n[1] + 2, n[0] + 2), dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        t = _median_ms(lambda: C.do_transform(x, y, kinds), reps)
+        V = n[0] * n[1] * n[2]
+        out["transform_ms"]["x".join(map(str, n))] = t
+        log(f"fp64 do_transform {n}: {t:.3f} ms ({48.0 * V / t / 1e6:.0f} GB/s model)")
+        del x, y
+    n, bc, lam = (512, 384, 256), (3, 4, 0), -4.0
+    V = n[0] * n[1] * n[2]
+    f_np, bcs = W.smooth_modes(n)
+    f0 = torch.from_numpy(f_np).to(dev)
+    del f_np
+    bct = [None if b is None else torch.from_numpy(b).to(dev) for b in bcs]
+    ref = f0.clone()
+    C.helmholtz_solve(ref, n, bc, lam, bct)
+    u = f0.clone()
+    ms = _median_ms(lambda: C.helmholtz_solve(u, n, bc, lam, bct), reps)
+    rect = {"fp64_ms": ms, "fp64_hbm_gbs_model": 80.0 * V / ms / 1e6}
+    for k in range(0, 4):
+        u.copy_(f0)
+        C.helmholtz_solve_mixed(u, n, bc, lam, bct, refine=k)
+        err = ((u - ref).abs().max() / ref.abs().max()).item()
+        if err <= 1e-10:
+            mms = _median_ms(lambda: C.helmholtz_solve_mixed(u, n, bc, lam, bct, refine=k), reps)
+            rect.update({"refine": k, "mixed_ms": mms, "mixed_rel_err_vs_fp64": err, "mixed_vs_fp64": ms / mms})
+            break
+    log(f"512x384x256 solve: fp64 {ms:.3f} ms ({rect['fp64_hbm_gbs_model']:.0f} GB/s model), "
+        f"fp32 + {rect.get('refine')} refinement(s) {rect.get('mixed_ms', float('nan')):.3f} ms")
+    out["solve_512x384x256"] = rect
+    del u, ref, f0
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_adjoint(dev, steps=3):
     """Differentiable solve: batch 16 of 128^3, codes (1,3,0), lambda = -2,
     L = 1/2 |u - u_target|^2; forward and adjoint backward timed."""
@@ -242,6 +307,7 @@ def main():
     ap.add_argument("--n", type=int, nargs=3, default=[512, 512, 512])
     ap.add_argument("--no-batch", action="store_true", help="skip the 2048-problem batched solve")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end hw3crt_ timing from host arrays")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the transforms and solves at the other sizes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-solve timing")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -380,6 +446,7 @@ def main():
 
     batched = None if args.no_batch else bench_batched(dev, max(1, min(args.steps, 10)))
     adjoint = None if args.no_adjoint else bench_adjoint(dev)
+    sizes = None if args.no_sizes else bench_sizes(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -412,6 +479,7 @@ def main():
                             "hw3crt_end_to_end_ms": e2e_ms, "pass_ms": avg},
                    "fp32_refine": mixed_info,
                    "transforms_per_s_fp64": 1000.0 / tms,
+                   "other_sizes": sizes,
                    "batched": batched,
                    "adjoint": adjoint},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "pass": dom, "achieved": achieved,
