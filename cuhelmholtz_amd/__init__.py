@@ -356,6 +356,12 @@ def helmholtz_adjoint_batch(u, gu, n, bc, lam, dom=(0.0, 1.0) * 3, faces=False, 
     return gf, gfaces, glam
 
 
+def release():
+    """Free the cached twiddle tables, workspaces and batched plans of the current
+    device (chz_release; the reference's freeMemory_cubic)."""
+    lib().chz_release()
+
+
 def set_timing(on):
     """Record per-pass events on every transform / solve (reference TIME_TEST counterpart)."""
     lib().chz_set_timing(1 if on else 0)

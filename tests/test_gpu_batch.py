@@ -248,3 +248,22 @@ def test_solve_batch_cache_evicts():
                 want = f[k].clone()
                 C.helmholtz_solve(want, n, (1, 1, 0), [-1.0, -2.0][k])
                 assert rel(u[k], want) <= SAME, (it, N, k)
+
+
+def test_release_then_solve_again():
+    """chz_release frees the cached plans and workspaces; later calls rebuild them"""
+    n, bc = (16, 12, 8), (1, 3, 0)
+    NX, NY, NZ = n
+    f = torch.randn((3, NZ + 1, NY + 1, NX + 1), dtype=torch.float64, device=dev())
+    face = torch.randn((NZ + 1) * (NX + 1), dtype=torch.float64, device=dev())
+    bcs = (None, None, face, face, None, None)
+    u1 = f.clone()
+    C.helmholtz_solve_batch(u1, n, bc, -1.5, bcs)
+    torch.cuda.synchronize()
+    C.release()
+    u2 = f.clone()
+    C.helmholtz_solve_batch(u2, n, bc, -1.5, bcs)
+    w = f[1].clone()
+    C.helmholtz_solve(w, n, bc, -1.5, [face if b is not None else None for b in bcs])
+    torch.cuda.synchronize()
+    assert rel(u2, u1) <= SAME and rel(u2[1], w) <= SAME
