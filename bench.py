@@ -301,8 +301,8 @@ def bench_adjoint(dev, steps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, nargs=3, default=[512, 512, 512])
     ap.add_argument("--no-batch", action="store_true", help="skip the 2048-problem batched solve")
