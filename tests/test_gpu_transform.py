@@ -166,6 +166,19 @@ def test_f32_path_matches_oracle():
         assert rel(got[sl], ref[sl]) < 2e-6, kinds
 
 
+@pytest.mark.parametrize("n,kinds", [((512, 256, 16), (O.DCT_2, O.DST_1, O.DFT_R2C)),
+                                     ((16, 384, 512), (O.DST_3, O.DCT_1, O.DCT_3)),
+                                     ((256, 512, 8), (O.DFT_R2C, O.DST_2, O.DCT_2))])
+def test_f32_path_large_lines(n, kinds):
+    """FP32 transforms on the lean-I/O shapes of N = 256, 384, 512 (rows, columns, z columns)"""
+    rng = np.random.default_rng(5)
+    v = rand_volume(rng, kinds, n)
+    ref = O.transform(v, *kinds)
+    got = gpu_transform(v, kinds, inplace=True, dtype=torch.float32)
+    sl = out_slices(kinds, n)
+    assert rel(got[sl], ref[sl]) < 5e-6, kinds
+
+
 def test_errors_are_reported():
     x = torch.zeros((8, 8, 9), dtype=torch.float64, device=dev())   # NX = 7 (odd)
     with pytest.raises(C.CHZError, match="even"):
