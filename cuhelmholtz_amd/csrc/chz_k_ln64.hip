@@ -1,7 +1,9 @@
 // FP64 fixed-shape pass kernels for N = 512 with lean tile I/O (chz_line.h):
 // 8 points per thread, 32 threads per line, 8 or 16 lines per tile, radix
-// plan 8 x 8 x 4; and for the column passes 16 points per thread, 16 threads
-// per line, 16 lines (whole 128-byte lines per slot), radix plan 16 x 16.
+// plan 8 x 8 x 4 (the tuned shapes: x passes 8 lines, column and fused z
+// passes 16 lines = whole 128-byte lines per slot on 512 threads); and, as
+// the measured alternative for the column passes, 16 points per thread, 16
+// threads per line, 16 lines, radix plan 16 x 16.
 #include "chz_line.h"
 namespace chz {
 
