@@ -163,7 +163,10 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     __device__ __forceinline__ static void stage_xres(const SolveArgs &sa, const double *U, V *buf, int nlines,
                                                       int a_tile, int k, int t)
     {
-        constexpr int RC = 4;   // chunks whose loads are issued together
+        // rows of 32+ lanes (a power of two): the x neighbours of the interior
+        // chunks come from the adjacent lanes (below); narrower rows load them
+        constexpr bool XSH = (TPL & (TPL - 1)) == 0 && TPL >= 32 && TPL <= 64;
+        constexpr int RC = XSH ? 2 : 4;   // chunks whose loads are issued together
         const int r = t / TPL, l = t % TPL;
         const bool rok = r < nlines;
         const int j = a_tile + (rok ? r : 0);
@@ -198,6 +201,23 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         const double pert = sa.pertrb ? sa.pertrb[0] * sa.pert_scale : 0.0;
         const double lam = sa.lambda, ix = sa.ih2[0], iy = sa.ih2[1], iz = sa.ih2[2];
         T *lp = lds_base(buf, r, l);
+        // XSH: U at this lane's slot of every chunk is loaded first; in the
+        // interior chunks the x neighbours are the adjacent lanes' values (the
+        // lanes of a row are one power-of-two segment of the wave), across a
+        // chunk edge the neighbouring chunk's end lane (every window contains
+        // slots 2..N-2 and xe >= N-1, so those values are the plain U of the slot)
+        double u0a[XSH ? NCH : 1];
+        if constexpr (XSH) {
+            static_for<0, NCH>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                if constexpr (interior(i)) {
+                    u0a[i] = ul[i * TPL];
+                } else {
+                    const int s = l + i * TPL;
+                    u0a[i] = ur[(rok && s >= xs && s <= xe) ? s : xs];
+                }
+            });
+        }
         static_for<0, (NCH + RC - 1) / RC>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             double fv[RC], u0[RC], um[RC], up[RC], uy[RC], uz[RC];
@@ -206,9 +226,17 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
                 if constexpr (i < NCH) {
                     if constexpr (interior(i)) {
                         fv[ii] = fl[i * TPL];
-                        u0[ii] = ul[i * TPL];
-                        um[ii] = ul[i * TPL - 1];
-                        up[ii] = ul[i * TPL + 1];
+                        if constexpr (XSH) {
+                            u0[ii] = u0a[i];
+                            const double nx = __shfl(u0a[i], l + 1, TPL), nx1 = __shfl(u0a[i + 1], l + 1, TPL);
+                            const double pv = __shfl(u0a[i], l - 1, TPL), pv1 = __shfl(u0a[i - 1], l - 1, TPL);
+                            up[ii] = l == TPL - 1 ? nx1 : nx;
+                            um[ii] = l == 0 ? pv1 : pv;
+                        } else {
+                            u0[ii] = ul[i * TPL];
+                            um[ii] = ul[i * TPL - 1];
+                            up[ii] = ul[i * TPL + 1];
+                        }
                         uy[ii] = wm[0] * uym[i * TPL] + wp[0] * uyp[i * TPL];
                         uz[ii] = wm[1] * uzm[i * TPL] + wp[1] * uzp[i * TPL];
                     } else {
@@ -216,7 +244,8 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
                         const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
                         const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
                         fv[ii] = fr[sc_];
-                        u0[ii] = ur[sc_];
+                        if constexpr (XSH) u0[ii] = u0a[i];
+                        else u0[ii] = ur[sc_];
                         um[ii] = ur[sm];
                         up[ii] = ur[sp];
                         uy[ii] = wm[0] * (ur + dm[0])[sc_] + wp[0] * (ur + dp[0])[sc_];
