@@ -128,8 +128,8 @@ def test_singular_zero_mode(bc):
 
 
 def test_manufactured_vs_hw3crt_and_second_order():
-    """BASELINE configuration 1 (64^3, codes (1,2,0), lambda -1) against the
-    hw3crt-contract CPU solver at 1e-12, and O(h^2) against the analytic u."""
+    """The manufactured-solution workload (64^3, codes (1,2,0), lambda -1) against
+    the hw3crt-contract CPU solver at 1e-12, and O(h^2) against the analytic u."""
     errs = []
     for N in (32, 64, 128):
         f, u, bcs = W.manufactured((N, N, N))

@@ -211,7 +211,7 @@ def test_ierror_codes():
 
 
 def manufactured(n):
-    """u = sin(pi x) sin(pi y/2) cos(2 pi z), codes
+    """The manufactured-solution workload: u = sin(pi x) sin(pi y/2) cos(2 pi z), codes
     (1,2,0), lambda = -1 on [0,1]^3; f = Delta u + lambda u at the nodes with
     Dirichlet values written into the boundary nodes (testgpu.cu:57-122)."""
     NX, NY, NZ = n
