@@ -440,7 +440,8 @@ class BatchSolver:
 
     def rebind(self, problems):
         """Point the plan at new problems of the same geometry (chz_batch_rebind):
-        new f / face tensors and lambdas; waits for the current stream."""
+        new f / face tensors and lambdas, uploaded on the current stream after
+        the plan's earlier solves (no host wait)."""
         import torch
         arr, keep, _ = self._problems(problems)
         with torch.cuda.device(self.device):

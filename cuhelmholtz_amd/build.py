@@ -7,6 +7,7 @@ Usage: python -m cuhelmholtz_amd.build  (or build() from Python).
 """
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -35,8 +36,24 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
+def _deps(path, seen=None):
+    """Local headers a source includes, transitively (#include "...")."""
+    seen = set() if seen is None else seen
+    with open(path) as f:
+        for m in re.finditer(r'#include "([^"]+)"', f.read()):
+            h = os.path.normpath(os.path.join(os.path.dirname(path), m.group(1)))
+            if h not in seen and os.path.exists(h):
+                seen.add(h)
+                _deps(h, seen)
+    return seen
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if all(os.path.getmtime(d) < t for d in [src, __file__, *_deps(src)]):
+            return obj
     cmd = [hipcc()] + COMMON
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + DEVICE
@@ -51,15 +68,19 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
-def build(verbose=True):
-    """Compile every HIP source for gfx950 and link the shared library."""
+def build(verbose=True, incremental=False):
+    """Compile every HIP source for gfx950 and link the shared library.
+
+    incremental: recompile only sources whose object is older than the source,
+    a local header it includes or this script (a development shortcut; the
+    default compiles everything)."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     # longest (kernel) files first so the pool stays busy
     srcs.sort(key=lambda s: -os.path.getsize(s))
     jobs = min(len(srcs), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -86,5 +107,5 @@ def build_drivers():
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

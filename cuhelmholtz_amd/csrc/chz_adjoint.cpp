@@ -34,22 +34,23 @@ static int adjoint(const double dom[6], const int N[3], const int bc[3], double 
     if (gu == gf) return fail(ERR_INVALID, "the output gradient must not alias the input gradient");
     if (glambda && !u) return fail(ERR_INVALID, "the lambda gradient needs the forward solution u");
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
+    WsLease ws(ctx, st);
     const long long fx = N[0] + 1, fy = N[1] + 1, fz = N[2] + 1;
     const long long nodes = fx * fy * fz;
     const long long maxface = std::max(fy * fz, std::max(fx * fz, fx * fy));
     // homogeneous boundary data for the adjoint solve: a zeroed face buffer
-    double *zero = nodes > 0 ? reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)maxface, 5)) : nullptr;
+    double *zero = nodes > 0 ? reinterpret_cast<double *>(ws.get(sizeof(double) * (size_t)maxface, 5)) : nullptr;
     const double *faces[6] = {zero, zero, zero, zero, zero, zero};
     SolveGeom g;
     int rc = make_geom(g, dom, N, bc, lambda, faces);
     if (rc) return rc;
     if (!zero) return fail(ERR_ALLOC, "workspace allocation failed");
     const size_t exbytes = sizeof(double) * (size_t)ws_row(N[0], sizeof(double)) * (N[1] + 2) * (N[2] + 2);
-    double *ex = reinterpret_cast<double *>(ctx.workspace(exbytes, 0));
-    double *gw = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)nodes, 4));
+    double *ex = reinterpret_cast<double *>(ws.get(exbytes, 0));
+    double *gw = reinterpret_cast<double *>(ws.get(sizeof(double) * (size_t)nodes, 4));
     const int nblk = 1024;
-    double *part = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * nblk, 6));
+    double *part = reinterpret_cast<double *>(ws.get(sizeof(double) * nblk, 6));
     if (!ex || !gw || !part) return fail(ERR_ALLOC, "workspace allocation failed");
     std::vector<PassSpec> v;
     if ((rc = solve_passes<double>(ctx, g, gw, ex, nullptr, nullptr, v)) || (rc = final_pass<double>(ctx, g, ex, nullptr, gw, v)))
@@ -128,14 +129,16 @@ static int adjoint_batch(const double dom[6], const int N[3], const int bc[3], i
     const long long maxface = std::max(fsz[0], std::max(fsz[1], fsz[2]));
     const int nblk = 1024;
     Context &ctx = Context::get();
-    double *zero, *gw, *part;
+    // held through the launches (the batched solve below re-enters it), so the
+    // workspace slots are ordered after their previous users
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
+    WsLease ws(ctx, st);
+    double *zero = reinterpret_cast<double *>(ws.get(sizeof(double) * (size_t)maxface, 5));
+    double *gw = reinterpret_cast<double *>(ws.get(sizeof(double) * (size_t)(nodes * count), 4));
+    double *part = reinterpret_cast<double *>(ws.get(sizeof(double) * (size_t)nblk * count, 6));
+    if (!zero || !gw || !part) return fail(ERR_ALLOC, "workspace allocation failed");
     SolveGeom g;
     {
-        std::lock_guard<std::mutex> lock(ctx.mutex());
-        zero = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)maxface, 5));
-        gw = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)(nodes * count), 4));
-        part = reinterpret_cast<double *>(ctx.workspace(sizeof(double) * (size_t)nblk * count, 6));
-        if (!zero || !gw || !part) return fail(ERR_ALLOC, "workspace allocation failed");
         const double *faces[6] = {zero, zero, zero, zero, zero, zero};
         int rc = make_geom(g, dom, N, bc, lambdas[0], faces);
         if (rc) return rc;

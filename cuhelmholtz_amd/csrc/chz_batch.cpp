@@ -66,12 +66,18 @@ struct GroupRec {
 struct BatchPlan {
     int dev = 0, count = 0, prec = 64, refine = 0;
     void *work = nullptr;     // per-problem workspaces
-    void *tables = nullptr;   // descriptors, items, periodic copies
+    void *tables = nullptr;   // device image: descriptors, periodic copies | items | twiddles, eigenvalues
     size_t work_bytes = 0, table_bytes = 0;
+    size_t desc_bytes = 0;                // descriptors and copy lists: the part a rebind rewrites
     std::vector<char> host;               // host image of the tables
+    char *staging = nullptr;              // pinned copy of the rewritable part (rebind uploads)
     std::vector<GroupRec> groups;
     size_t c_off[3] = {0, 0, 0};          // periodic copy lists in the image
     std::vector<size_t> ws_off, u_off;    // workspace / iterate offsets in work
+    // the plan's own twiddle and eigenvalue tables (image offsets): the
+    // descriptors point here, never at the library's caches
+    std::map<std::pair<int, int>, size_t> tw_off;   // (N, precision)
+    std::vector<size_t> eig_off;                    // per problem
     std::vector<ProbKey> keys;
     std::vector<BatchLaunch> launches;
     const BatchCopy *copies[3] = {nullptr, nullptr, nullptr};
@@ -79,13 +85,32 @@ struct BatchPlan {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // stream order across the streams a plan is used on: `done` marks the
+    // end of the last solve or upload enqueued for the plan; `uploaded` the
+    // last rebind upload (the staging buffer is free again once it has run)
+    std::mutex mu;
+    hipEvent_t done = nullptr, uploaded = nullptr;
+    bool done_rec = false, up_rec = false;
     ~BatchPlan()
     {
+        if (done && done_rec) (void)hipEventSynchronize(done);   // the plan may still run on some stream
+        if (uploaded && up_rec) (void)hipEventSynchronize(uploaded);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
         if (tables) (void)hipFree(tables);
         if (work) (void)hipFree(work);
+        if (staging) (void)hipHostFree(staging);
+        if (done) (void)hipEventDestroy(done);
+        if (uploaded) (void)hipEventDestroy(uploaded);
+    }
+    // order st after the plan's previous use (no host wait)
+    hipError_t wait_on(hipStream_t st) { return done_rec ? hipStreamWaitEvent(st, done, 0) : hipSuccess; }
+    hipError_t mark(hipStream_t st)
+    {
+        hipError_t e = hipEventRecord(done, st);
+        done_rec = done_rec || e == hipSuccess;
+        return e;
     }
 };
 
@@ -169,16 +194,25 @@ static int build_specs(Context &ctx, const chz_problem *probs, const BatchPlan &
     return OK;
 }
 
-// descriptors of every group and the periodic copy lists into the table image
+// descriptors of every group and the periodic copy lists into the table
+// image; the twiddle and eigenvalue pointers of the descriptors are redirected
+// to the plan's own copies (device base of the plan's table allocation)
 static void write_descs(BatchPlan &plan, const chz_problem *probs, const std::vector<SolveGeom> &geo,
-                        const std::vector<std::vector<PassSpec>> &specs)
+                        const std::vector<std::vector<PassSpec>> &specs, char *image)
 {
+    char *tdev = reinterpret_cast<char *>(plan.tables);
     for (const GroupRec &gr : plan.groups) {
-        BatchDesc *d = reinterpret_cast<BatchDesc *>(plan.host.data() + gr.d_off);
-        for (size_t k = 0; k < gr.probs.size(); k++) d[k] = specs[gr.probs[k]][gr.step].d;
+        BatchDesc *d = reinterpret_cast<BatchDesc *>(image + gr.d_off);
+        for (size_t k = 0; k < gr.probs.size(); k++) {
+            const int i = gr.probs[k];
+            const PassSpec &q = specs[i][gr.step];
+            d[k] = q.d;
+            d[k].p.tw = tdev + plan.tw_off.at(std::make_pair(q.N, q.prec));
+            if (d[k].sa.eig) d[k].sa.eig = reinterpret_cast<const double *>(tdev + plan.eig_off[i]);
+        }
     }
     for (int a = 0; a < 3; a++) {
-        BatchCopy *c = reinterpret_cast<BatchCopy *>(plan.host.data() + plan.c_off[a]);
+        BatchCopy *c = reinterpret_cast<BatchCopy *>(image + plan.c_off[a]);
         int n = 0;
         for (int i = 0; i < plan.count; i++)
             if (geo[i].bc[a] == 0) c[n++] = BatchCopy{probs[i].f, geo[i].N[0], geo[i].N[1], geo[i].N[2], a};
@@ -194,7 +228,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     if (prec == 64 && refine != 0) return fail(ERR_INVALID, "the FP64 batched solve takes no refinement steps");
     if (refine < 0 || refine > 16) return fail(ERR_INVALID, "refinement count must be in 0..16");
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
 
     std::vector<SolveGeom> geo(count);
     const size_t esz = prec == 64 ? 8 : 4;
@@ -262,7 +296,9 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
             }
         }
     }
-    // table layout: per group descriptors then items
+    // table layout: descriptors of every group and the periodic copy lists
+    // (rewritten by a rebind), then the work items, then the plan's own
+    // twiddle and eigenvalue tables
     std::vector<size_t> i_off(groups.size());
     std::vector<int> nitems(groups.size());
     size_t tb = 0;
@@ -280,8 +316,6 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         nitems[gi] = (int)n;
         gr.d_off = tb;
         tb += align256(sizeof(BatchDesc) * gr.probs.size());
-        i_off[gi] = tb;
-        tb += align256(sizeof(BatchItem) * (size_t)n);
     }
     for (int a = 0; a < 3; a++) {
         int maxface = 0, n = 0;
@@ -302,9 +336,56 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         plan->c_off[a] = tb;
         tb += align256(sizeof(BatchCopy) * (size_t)n);
     }
+    plan->desc_bytes = tb;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        i_off[gi] = tb;
+        tb += align256(sizeof(BatchItem) * (size_t)nitems[gi]);
+    }
+    // twiddles per (N, precision) and eigenvalues per distinct geometry
+    std::vector<std::pair<size_t, std::vector<char>>> blobs;
+    for (int i = 0; i < count; i++)
+        for (const PassSpec &q : specs[i]) {
+            const auto key = std::make_pair(q.N, q.prec);
+            if (plan->tw_off.count(key)) continue;
+            plan->tw_off[key] = tb;
+            blobs.push_back({tb, twiddle_bytes(q.N, q.prec)});
+            tb += align256(blobs.back().second.size());
+        }
+    {
+        std::map<std::string, size_t> seen;
+        plan->eig_off.assign(count, 0);
+        for (int i = 0; i < count; i++) {
+            const std::vector<double> t = eig_tables(geo[i]);
+            std::string k(reinterpret_cast<const char *>(t.data()), t.size() * sizeof(double));
+            auto it = seen.find(k);
+            if (it != seen.end()) {
+                plan->eig_off[i] = it->second;
+                continue;
+            }
+            plan->eig_off[i] = seen[k] = tb;
+            blobs.push_back({tb, std::vector<char>(k.begin(), k.end())});
+            tb += align256(k.size());
+        }
+    }
+    plan->table_bytes = tb;
+    if (hipMalloc(&plan->tables, tb) != hipSuccess) {
+        delete plan;
+        return fail(ERR_ALLOC, "batched descriptor table allocation failed");
+    }
+    if (hipHostMalloc((void **)&plan->staging, plan->desc_bytes, hipHostMallocDefault) != hipSuccess) {
+        plan->staging = nullptr;
+        delete plan;
+        return fail(ERR_ALLOC, "batched descriptor staging allocation failed");
+    }
+    if (hipEventCreateWithFlags(&plan->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&plan->uploaded, hipEventDisableTiming) != hipSuccess) {
+        delete plan;
+        return fail(ERR_DEVICE, "batched plan event creation failed");
+    }
     std::vector<char> &host = plan->host;
     host.assign(tb, 0);
-    write_descs(*plan, probs, geo, specs);
+    for (const auto &b : blobs) memcpy(host.data() + b.first, b.second.data(), b.second.size());
+    write_descs(*plan, probs, geo, specs, host.data());
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const GroupRec &gr = groups[gi];
         BatchItem *it = reinterpret_cast<BatchItem *>(host.data() + i_off[gi]);
@@ -323,15 +404,11 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
             }
         }
     }
-    plan->table_bytes = tb;
-    if (hipMalloc(&plan->tables, tb) != hipSuccess) {
-        delete plan;
-        return fail(ERR_ALLOC, "batched descriptor table allocation failed");
-    }
     if (hipMemcpy(plan->tables, host.data(), tb, hipMemcpyHostToDevice) != hipSuccess) {
         delete plan;
         return fail(ERR_DEVICE, "batched descriptor table upload failed");
     }
+    std::vector<char>().swap(host);   // a rebind writes the descriptors into the staging buffer
     char *tbase = reinterpret_cast<char *>(plan->tables);
     for (int a = 0; a < 3; a++) plan->copies[a] = reinterpret_cast<const BatchCopy *>(tbase + plan->c_off[a]);
 
@@ -397,15 +474,20 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
 
 // Point a plan at a new problem list of the same geometry (sizes, codes,
 // domains, which face arrays, singularity): new f / face arrays and lambdas.
-// The descriptors and periodic copy lists are rebuilt and uploaded on st after
-// the work already queued there (the items, workspaces and graph stay).
-static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStream_t st)
+// The descriptors and periodic copy lists are rebuilt on the host and
+// uploaded on st from the plan's pinned staging buffer, ordered after the
+// plan's earlier solves on any stream (the items, tables, workspaces and graph
+// stay).  The host waits only for the plan's previous upload to have read the
+// staging buffer.  *cfg_changed is set when the launch configurations changed
+// since the plan was made (tuning calls): the plan must then be rebuilt.
+static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStream_t st, bool *cfg_changed = nullptr)
 {
+    if (cfg_changed) *cfg_changed = false;
     if (!probs || count != plan.count) return fail(ERR_INVALID, "rebind: the problem count differs from the plan's");
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev != plan.dev) return fail(ERR_INVALID, "plan belongs to another device");
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
     std::vector<SolveGeom> geo(count);
     for (int i = 0; i < count; i++) {
         const chz_problem &q = probs[i];
@@ -421,18 +503,46 @@ static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStrea
     for (const GroupRec &gr : plan.groups)
         for (int i : gr.probs) {
             const PassSpec &q = specs[i][gr.step];
-            if (std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan) != gr.key)
+            if (std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan) != gr.key) {
+                if (cfg_changed) *cfg_changed = true;
                 return fail(ERR_INVALID, "rebind: launch configurations changed since the plan was made");
+            }
         }
-    // the image is rewritten only after the stream's earlier uploads have run
-    hipError_t e = hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> plock(plan.mu);
+    hipError_t e = hipSuccess;
+    if (plan.up_rec) e = hipEventSynchronize(plan.uploaded);   // staging buffer free again
     if (e == hipSuccess) {
-        write_descs(plan, probs, geo, specs);
-        e = hipMemcpyAsync(plan.tables, plan.host.data(), plan.table_bytes, hipMemcpyHostToDevice, st);
+        write_descs(plan, probs, geo, specs, plan.staging);
+        e = plan.wait_on(st);   // earlier solves of the plan have read the old descriptors
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipMemcpyAsync(plan.tables, plan.staging, plan.desc_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        e = hipEventRecord(plan.uploaded, st);
+        plan.up_rec = plan.up_rec || e == hipSuccess;
+    }
+    if (e == hipSuccess) e = plan.mark(st);
     if (e != hipSuccess) return fail(ERR_DEVICE, std::string("rebind upload: ") + hipGetErrorString(e));
     return OK;
+}
+
+// Replay the plan's graph (or issue its launches) on st after its previous use.
+static int launch_plan(Context &ctx, BatchPlan &plan, bool use_graph, hipStream_t st)
+{
+    std::lock_guard<std::mutex> plock(plan.mu);
+    hipError_t e = plan.wait_on(st);
+    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("plan stream wait: ") + hipGetErrorString(e));
+    int rc = OK;
+    if (use_graph) {
+        e = hipGraphLaunch(plan.exec, st);
+        if (e != hipSuccess) rc = fail(ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+    } else {
+        std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
+        const bool tm = ctx.timing();
+        rc = enqueue(ctx, plan, st, tm);
+        if (tm) ctx.timing_end(st);
+    }
+    if ((e = plan.mark(st)) != hipSuccess && rc == OK) rc = fail(ERR_DEVICE, std::string("plan event: ") + hipGetErrorString(e));
+    return rc;
 }
 
 // Plans kept for repeated solves of one geometry (chz_batch_solve_cached):
@@ -449,13 +559,11 @@ static PlanCache &plan_cache()
     return *c;
 }
 
-// drop every cached plan (chz_release)
+// drop every cached plan (chz_release); each waits for its own last use
 void release_cached_plans()
 {
     PlanCache &c = plan_cache();
     std::lock_guard<std::mutex> lock(c.mu);
-    if (c.plans.empty()) return;
-    (void)hipDeviceSynchronize();
     for (auto &e : c.plans) delete e.second;
     c.plans.clear();
 }
@@ -489,6 +597,9 @@ static int solve_cached(const chz_problem *probs, int count, int prec, int refin
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return fail(ERR_DEVICE, "no current device");
     const std::string sig = signature(probs, count, prec, refine, dev);
+    // lock order: the context, then the cache (callers holding the context re-enter it)
+    Context &ctx = Context::get();
+    std::lock_guard<std::recursive_mutex> clock(ctx.mutex());
     PlanCache &c = plan_cache();
     std::lock_guard<std::mutex> lock(c.mu);
     BatchPlan *plan = nullptr;
@@ -500,24 +611,22 @@ static int solve_cached(const chz_problem *probs, int count, int prec, int refin
         }
     int rc;
     if (plan) {
-        if ((rc = rebind(*plan, probs, count, st))) {
-            (void)hipDeviceSynchronize();   // the dropped plan may still run on another stream
-            delete plan;
-            return rc;
+        bool cfg_changed = false;
+        if ((rc = rebind(*plan, probs, count, st, &cfg_changed))) {
+            delete plan;   // waits for its last use
+            plan = nullptr;
+            if (!cfg_changed) return rc;
         }
-    } else {
+    }
+    if (!plan) {
         if ((rc = create(probs, count, prec, refine, &plan))) return rc;
         if (c.plans.size() >= kCached) {
-            // the evicted plan may still run on some stream
-            (void)hipDeviceSynchronize();
-            delete c.plans.front().second;
+            delete c.plans.front().second;   // waits for its last use
             c.plans.erase(c.plans.begin());
         }
     }
     c.plans.push_back({sig, plan});
-    hipError_t e = hipGraphLaunch(plan->exec, st);
-    if (e != hipSuccess) return fail(ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
-    return OK;
+    return launch_plan(ctx, *plan, true, st);
 }
 
 }  // namespace chz
@@ -540,18 +649,7 @@ int chz_batch_solve(void *plan, int use_graph, void *stream)
     BatchPlan *p = reinterpret_cast<BatchPlan *>(plan);
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev != p->dev) return chz::fail(chz::ERR_INVALID, "plan belongs to another device");
-    hipStream_t st = (hipStream_t)stream;
-    if (use_graph) {
-        hipError_t e = hipGraphLaunch(p->exec, st);
-        if (e != hipSuccess) return chz::fail(chz::ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
-        return chz::OK;
-    }
-    chz::Context &ctx = chz::Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
-    const bool tm = ctx.timing();
-    int rc = chz::enqueue(ctx, *p, st, tm);
-    if (tm) ctx.timing_end(st);
-    return rc;
+    return chz::launch_plan(chz::Context::get(), *p, use_graph != 0, (hipStream_t)stream);
 }
 
 int chz_batch_rebind(void *plan, const chz_problem *problems, int count, void *stream)

@@ -24,10 +24,14 @@ int fail(int code, const std::string &msg)
 const char *last_error() { return g_err.c_str(); }
 
 // Radix sequence for an M-point FFT with TPL threads per line and at most ept
-// points per thread: every stage R must satisfy ceil(M/(R*TPL)) <= ept/R.
+// points per thread: every stage R with a specialised butterfly must satisfy
+// ceil(M/(R*TPL)) <= ept/R.  Factors without one (primes from 7 up) become
+// direct stages (fft_stage_any: byte R for R <= 255, byte 1 for a remaining
+// factor whose smallest prime is larger), which need only M <= ept * TPL.
 bool radix_plan(int M, int ept, int tpl, unsigned long long *plan)
 {
     static const int radices[] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
+    if (M < 1 || M > ept * tpl) return false;
     unsigned long long p = 0;
     int rem = M, st = 0;
     while (rem > 1) {
@@ -39,8 +43,15 @@ bool radix_plan(int M, int ept, int tpl, unsigned long long *plan)
             pick = R;
             break;
         }
-        if (!pick || st >= 8) return false;
-        p |= (unsigned long long)pick << (8 * st++);
+        if (!pick) {
+            int q = 2;
+            while (q * q <= rem && rem % q) q++;
+            if (rem % q) q = rem;   // rem is prime
+            if (q < 7) return false;   // a 2, 3 or 5 that does not fit this thread shape
+            pick = q <= 255 ? q : rem;
+        }
+        if (st >= 8) return false;
+        p |= (unsigned long long)(pick <= 255 ? pick : 1) << (8 * st++);
         rem /= pick;
     }
     *plan = p;
@@ -117,11 +128,8 @@ Context &Context::get()
 
 // Twiddles are generated in FP64 with the host libm and rounded once to the
 // pass precision (the FP32 path uses FP64-generated, rounded twiddles).
-const void *Context::twiddles(int N, int prec)
+std::vector<char> twiddle_bytes(int N, int prec)
 {
-    auto key = std::make_pair(N, prec);
-    auto it = tw_.find(key);
-    if (it != tw_.end()) return it->second;
     const int M = N / 2;
     const int n = TwOffsets::total(N);
     std::vector<double> h(2 * (size_t)n, 0.0);
@@ -140,18 +148,25 @@ const void *Context::twiddles(int N, int prec)
         double a = pi * (double)k / (double)N;
         h[2 * (TwOffsets::rsin(N) + k)] = k == 0 ? 0.0 : 1.0 / std::sin(a);
     }
-    void *d = nullptr;
-    size_t bytes = h.size() * (prec == 64 ? 8 : 4);
-    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
-    hipError_t e;
+    std::vector<char> out(h.size() * (prec == 64 ? 8 : 4));
     if (prec == 64) {
-        e = hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice);
+        memcpy(out.data(), h.data(), out.size());
     } else {
-        std::vector<float> f(h.size());
+        float *f = reinterpret_cast<float *>(out.data());
         for (size_t i = 0; i < h.size(); i++) f[i] = (float)h[i];
-        e = hipMemcpy(d, f.data(), bytes, hipMemcpyHostToDevice);
     }
-    if (e != hipSuccess) {
+    return out;
+}
+
+const void *Context::twiddles(int N, int prec)
+{
+    auto key = std::make_pair(N, prec);
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second;
+    const std::vector<char> h = twiddle_bytes(N, prec);
+    void *d = nullptr;
+    if (hipMalloc(&d, h.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return nullptr;
     }
@@ -215,31 +230,63 @@ bool Context::config(int N, int prec, int role, AxisCfg *out)
     return true;
 }
 
-void *Context::workspace(size_t bytes, int slot)
+void *Context::workspace(size_t bytes, int slot, hipStream_t st)
 {
-    auto it = ws_.find(slot);
-    if (it != ws_.end() && it->second.second >= bytes) return it->second.first;
-    if (it != ws_.end()) {
-        (void)hipFree(it->second.first);
-        ws_.erase(it);
+    Slot &w = ws_[slot];
+    if (!w.done && hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
+        w.done = nullptr;
+        return nullptr;
     }
+    if (w.p && w.bytes >= bytes) {
+        // stream order after the slot's last user (a no-op on the same stream)
+        if (w.recorded && hipStreamWaitEvent(st, w.done, 0) != hipSuccess) return nullptr;
+        return w.p;
+    }
+    if (w.p) {
+        // the last user may still run on another stream
+        if (w.recorded) (void)hipEventSynchronize(w.done);
+        (void)hipFree(w.p);
+        w.p = nullptr;
+        w.bytes = 0;
+    }
+    w.recorded = false;
     void *d = nullptr;
     if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
-    ws_[slot] = std::make_pair(d, bytes);
+    w.p = d;
+    w.bytes = bytes;
     return d;
+}
+
+void Context::workspace_done(int slot, hipStream_t st)
+{
+    auto it = ws_.find(slot);
+    if (it == ws_.end() || !it->second.done) return;
+    it->second.recorded = hipEventRecord(it->second.done, st) == hipSuccess;
 }
 
 void *Context::cached_blob(const std::string &key, const void *data, size_t bytes)
 {
     auto it = blobs_.find(key);
-    if (it != blobs_.end()) return it->second;
+    if (it != blobs_.end()) {
+        it->second.second = ++blob_clock_;
+        return it->second.first;
+    }
+    if (blobs_.size() >= kMaxBlobs) {
+        // drop the least recently used table once no queued work can read it
+        auto lru = blobs_.begin();
+        for (auto jt = blobs_.begin(); jt != blobs_.end(); ++jt)
+            if (jt->second.second < lru->second.second) lru = jt;
+        (void)hipDeviceSynchronize();
+        (void)hipFree(lru->second.first);
+        blobs_.erase(lru);
+    }
     void *d = nullptr;
     if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
     if (hipMemcpy(d, data, bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return nullptr;
     }
-    blobs_[key] = d;
+    blobs_[key] = std::make_pair(d, ++blob_clock_);
     return d;
 }
 
@@ -277,13 +324,18 @@ int Context::timing_read(double *ms, int max, std::string *names)
 
 void Context::release()
 {
-    for (auto &kv : blobs_) (void)hipFree(kv.second);
+    std::lock_guard<std::recursive_mutex> lock(mu_);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : blobs_) (void)hipFree(kv.second.first);
     blobs_.clear();
     for (auto &kv : tw_) (void)hipFree(kv.second);
     tw_.clear();
-    for (auto &kv : ws_) (void)hipFree(kv.second.first);
-    ws_.clear();
-    cfg_.clear();
+    for (auto &kv : ws_) {
+        if (kv.second.p) (void)hipFree(kv.second.p);
+        kv.second.p = nullptr;
+        kv.second.bytes = 0;
+        kv.second.recorded = false;
+    }
 }
 
 }  // namespace chz

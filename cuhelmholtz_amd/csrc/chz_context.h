@@ -43,12 +43,21 @@ public:
     int load_tuning(const char *path);
     size_t lds_cap() const { return lds_cap_; }
     int device() const { return dev_; }
+    // wait for the device, then free the twiddle tables, workspaces and
+    // cached tables (the launch configurations are kept)
     void release();
-    // persistent workspace (grown on demand, kept until release())
-    void *workspace(size_t bytes, int slot = 0);
-    // device copy of a small host table, cached by its bytes (kept until release())
+    // persistent workspace `slot` of at least `bytes` (grown on demand, kept
+    // until release()), ordered on stream st after the last call that used
+    // the slot: st waits for that call's end event (no host wait; growing the
+    // slot waits on the host before freeing the old memory).  The caller
+    // marks its own end with workspace_done() after its last launch.
+    void *workspace(size_t bytes, int slot, hipStream_t st);
+    void workspace_done(int slot, hipStream_t st);
+    // device copy of a small host table, cached by its bytes (at most
+    // kMaxBlobs kept; the least recently used is freed after a device wait)
     void *cached_blob(const std::string &key, const void *data, size_t bytes);
-    std::mutex &mutex() { return mu_; }
+    static const size_t kMaxBlobs = 64;
+    std::recursive_mutex &mutex() { return mu_; }
     // per-pass timing (the counterpart of the reference's TIME_TEST splits):
     // when enabled, every pass of a transform / solve records an event at its
     // start and each call one at its end; events accumulate until read.
@@ -63,15 +72,43 @@ private:
     explicit Context(int dev);
     int dev_;
     size_t lds_cap_ = 65536;
-    std::mutex mu_;
+    std::recursive_mutex mu_;   // held while a call enqueues (re-entered by nested calls)
     std::map<std::pair<int, int>, void *> tw_;
     std::map<std::tuple<int, int, int>, AxisCfg> cfg_;
-    std::map<int, std::pair<void *, size_t>> ws_;
-    std::map<std::string, void *> blobs_;
+    struct Slot {
+        void *p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;   // end of the last call that used the slot
+        bool recorded = false;
+    };
+    std::map<int, Slot> ws_;
+    std::map<std::string, std::pair<void *, unsigned long long>> blobs_;   // device copy, last use
+    unsigned long long blob_clock_ = 0;
     bool timing_ = false;
     std::vector<hipEvent_t> ev_;
     int nev_ = 0;
     std::vector<std::string> names_;
+};
+
+// The workspace slots one call uses: acquired in order on the call's stream,
+// released (end events recorded) when the lease goes out of scope, after the
+// call's launches.
+struct WsLease {
+    Context &ctx;
+    hipStream_t s;
+    int slots[8];
+    int n = 0;
+    WsLease(Context &c, hipStream_t st) : ctx(c), s(st) {}
+    void *get(size_t bytes, int slot)
+    {
+        void *p = ctx.workspace(bytes, slot, s);
+        if (p && n < 8) slots[n++] = slot;
+        return p;
+    }
+    ~WsLease()
+    {
+        for (int i = 0; i < n; i++) ctx.workspace_done(slots[i], s);
+    }
 };
 
 // records pass boundaries while the context's timing is on
@@ -82,6 +119,9 @@ struct PassTimer {
     void mark(const char *name) { if (ctx.timing()) ctx.timing_mark(name, s); }
     ~PassTimer() { if (ctx.timing()) ctx.timing_end(s); }
 };
+
+// host image of the twiddle table of N (layout TwOffsets) at precision prec
+std::vector<char> twiddle_bytes(int N, int prec);
 
 // radix plan of an M-point FFT for tpl threads of at most ept points each
 bool radix_plan(int M, int ept, int tpl, unsigned long long *plan);

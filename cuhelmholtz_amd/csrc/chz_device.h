@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace chz {
 
 template <typename T> struct vec2_of;
@@ -238,15 +240,80 @@ __device__ __forceinline__ void fft_stage(const TL &tl, int u, int TPL, int M, i
     __syncthreads();
 }
 
+// One Stockham stage of any radix R as a direct DFT (the factors without a
+// specialised butterfly: primes from 7 up, or the whole remaining factor).
+// Output q of butterfly b is sum_r x_{b + r M/R} W_{Ns R}^{r j} W_R^{r q}
+// (j = b mod Ns) = sum_r x_{b + r M/R} W_M^{r (j M/(Ns R) + q M/R)}: every
+// factor comes from the M-point twiddle table.  A thread computes outputs
+// o = u + i TPL (o = b + q M/R) straight from LDS into registers, so any R
+// fits as long as M <= EPT * TPL; the work is M * R multiply-adds per line.
+template <int EPT, typename V, typename PT>
+__device__ __forceinline__ void stage_any_body(PT pt, int u, int TPL, int M, int Ns, int R, const V *__restrict__ tw)
+{
+    const int MR = M / R, twstride = MR / Ns;
+    V out[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int o = u + i * TPL;
+        if (o < M) {
+            const int b = o % MR, q = o / MR, j = b % Ns;
+            const int step = j * twstride + q * MR;   // < M
+            V acc = mk<V>(0, 0);
+            int idx = 0;
+            for (int r = 0; r < R; r++) {
+                const V x = pt(b + r * MR), w = tw[idx];
+                acc = mk<V>(acc.x + x.x * w.x - x.y * w.y, acc.y + x.x * w.y + x.y * w.x);
+                idx += step;
+                if (idx >= M) idx -= M;
+            }
+            out[i] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int o = u + i * TPL;
+        if (o < M) {
+            const int b = o % MR, q = o / MR, j = b % Ns;
+            pt((b - j) * R + j + q * Ns) = out[i];
+        }
+    }
+    __syncthreads();
+}
+
+// The direct stage on a plain tile (point k of line c at buf[k LS + c]), kept
+// out of line: the kernels with run-time radix plans reach it from every
+// inlined transform, and only sizes with large prime factors take it.
+template <int EPT, typename V>
+__device__ __noinline__ void fft_stage_any_plain(V *buf, int LS, int c, int u, int TPL, int M, int Ns, int R,
+                                                 const V *__restrict__ tw)
+{
+    stage_any_body<EPT, V>([=](int k) -> V & { return buf[k * LS + c]; }, u, TPL, M, Ns, R, tw);
+}
+
+template <int EPT, typename TL>
+__device__ __forceinline__ void fft_stage_any(const TL &tl, int u, int TPL, int M, int Ns, int R,
+                                              const typename TL::V *__restrict__ tw)
+{
+    typedef typename TL::V V;
+    if constexpr (std::is_same<TL, Tile<decltype(V().x)>>::value)
+        fft_stage_any_plain<EPT, V>(tl.buf, tl.LS, tl.c, u, TPL, M, Ns, R, tw);
+    else
+        stage_any_body<EPT, V>([&](int k) -> V & { return tl.pt(k); }, u, TPL, M, Ns, R, tw);
+}
+
 // Full forward FFT: stages described by a packed radix list (one byte per
-// stage, first stage in the low byte, 0 terminates).
+// stage, first stage in the low byte, 0 terminates).  Bytes 2, 3, 4, 5, 6, 8,
+// 10, 12, 16 are the specialised butterflies; any other byte R > 1 is a
+// direct stage of radix R, and byte 1 a direct stage of the whole remaining
+// factor M / (product of the earlier stages).
 template <int EPT, typename T, typename TL>
 __device__ __forceinline__ void fft_forward(const TL &tl, int u, int TPL, int M, unsigned long long plan,
                                             const typename vec2_of<T>::type *__restrict__ tw)
 {
     int Ns = 1;
     for (int st = 0; st < 8; st++) {
-        const int R = (int)((plan >> (8 * st)) & 255);
+        int R = (int)((plan >> (8 * st)) & 255);
         if (R == 0) break;
         switch (R) {
             case 2: if constexpr (EPT >= 2) fft_stage<2, EPT>(tl, u, TPL, M, Ns, tw); break;
@@ -258,7 +325,10 @@ __device__ __forceinline__ void fft_forward(const TL &tl, int u, int TPL, int M,
             case 10: if constexpr (EPT >= 10) fft_stage<10, EPT>(tl, u, TPL, M, Ns, tw); break;
             case 12: if constexpr (EPT >= 12) fft_stage<12, EPT>(tl, u, TPL, M, Ns, tw); break;
             case 16: if constexpr (EPT >= 16) fft_stage<16, EPT>(tl, u, TPL, M, Ns, tw); break;
-            default: break;
+            default:
+                if (R == 1) R = M / Ns;
+                fft_stage_any<EPT>(tl, u, TPL, M, Ns, R, tw);
+                break;
         }
         Ns *= R;
     }

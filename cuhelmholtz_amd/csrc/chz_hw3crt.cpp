@@ -10,7 +10,7 @@
 // IERROR as documented for FISHPACK's hw3crt: 1 xs >= xf, 2 L < 3, 3 LBDCND
 // outside 0..4, 4 ys >= yf, 5 M < 3, 6 MBDCND, 7 zs >= zf, 8 N < 3, 9 NBDCND,
 // 10 LDIMF < L+1, 11 MDIMF < M+1, 12 ELMBDA > 0 (a warning: the solution is
-// still computed); plus 13 for a cell count the GPU transforms do not support
+// still computed); plus 13 for an odd cell count (the GPU transforms take even ones)
 // (odd, or N/2 not of the form 2^a 3^b 5^c) and 20 for a device failure.
 // PERTRB: for ELMBDA = 0 with periodic / Neumann codes only the problem is
 // singular; PERTRB is the weighted mean of the folded rhs (the constant that
@@ -29,14 +29,7 @@
 
 namespace chz {
 
-static bool supported_cells(int n)
-{
-    if (n < 2 || (n & 1)) return false;
-    int m = n / 2;
-    for (int p : {2, 3, 5})
-        while (m % p == 0) m /= p;
-    return m == 1;
-}
+static bool supported_cells(int n) { return n >= 2 && !(n & 1); }
 
 // device buffers of one call, released on every exit path
 struct DevBufs {
@@ -71,7 +64,7 @@ static int hw3crt_gpu(double xs, double xf, int l, int lbdcnd, const double *bdx
     if (ldimf < l + 1) return 10;
     if (mdimf < m + 1) return 11;
     if (!supported_cells(l) || !supported_cells(m) || !supported_cells(n)) {
-        set_error("hw3crt_: cell counts must be even with N/2 = 2^a 3^b 5^c");
+        set_error("hw3crt_: cell counts must be even (the real-FFT split of the spectral solve)");
         return 13;
     }
     const int NX = l, NY = m, NZ = n;

@@ -78,7 +78,7 @@ int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3]
 
 // Slot eigenvalue tables (2 cos(m theta + shift) - 2)/h^2 of the three axes,
 // host FP64 (cuda_helmholtz_solver.cu:58-73, _cuda_helmholtz_.inl:413-418).
-static std::vector<double> eig_tables(const SolveGeom &g)
+std::vector<double> eig_tables(const SolveGeom &g)
 {
     const double pi = 3.14159265358979323846;
     std::vector<double> t;
@@ -148,15 +148,16 @@ static int check_spec(const PassSpec &q, long long svol, long long dvol, const c
 // refinement step of the mixed-precision solve).
 template <typename T>
 int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
-                 std::vector<PassSpec> &out)
+                 std::vector<PassSpec> &out, const SolveTables *tabs)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx, cy, cz;
     if (!ctx.config(NX, prec, ROLE_XFWD, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) || !ctx.config(NZ, prec, ROLE_ZMID, &cz))
-        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
-    const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
-    const double *eig = solver_eig(ctx, g);
+        return fail(ERR_UNSUPPORTED, "no launch configuration for this cell count (LDS or thread limits)");
+    const void *twx = tabs ? tabs->tw[0] : ctx.twiddles(NX, prec), *twy = tabs ? tabs->tw[1] : ctx.twiddles(NY, prec),
+               *twz = tabs ? tabs->tw[2] : ctx.twiddles(NZ, prec);
+    const double *eig = tabs ? tabs->eig : solver_eig(ctx, g);
     if (!twx || !twy || !twz || !eig) return fail(ERR_ALLOC, "table allocation failed");
     if (U && prec != 32) return fail(ERR_INVALID, "residual pass is FP32 only");
 
@@ -258,13 +259,14 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
 // into f over the unknown box (out == f, Uin == nullptr).  FP32: out = Uin +
 // ratio * delta in FP64 (Uin == nullptr: out = ratio * delta).
 template <typename T>
-int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v)
+int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v,
+               const SolveTables *tabs)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx;
     if (!ctx.config(NX, prec, ROLE_ROWS, &cx)) return fail(ERR_UNSUPPORTED, "unsupported cell count");
-    const void *twx = ctx.twiddles(NX, prec);
+    const void *twx = tabs ? tabs->tw[0] : ctx.twiddles(NX, prec);
     if (!twx) return fail(ERR_ALLOC, "table allocation failed");
     const long long fx = NX + 1, fy = NY + 1, P0 = ws_row(NX, sizeof(T)), P1 = NY + 2;
     const long long fvol = fx * fy * (NZ + 1), vol = P0 * P1 * (NZ + 2);
@@ -335,11 +337,13 @@ static int run_specs(Context &ctx, const std::vector<PassSpec> &v, hipStream_t s
 }
 
 template int solve_passes<double>(Context &, const SolveGeom &, double *, double *, const double *, const double *,
-                                  std::vector<PassSpec> &);
+                                  std::vector<PassSpec> &, const SolveTables *);
 template int solve_passes<float>(Context &, const SolveGeom &, double *, float *, const double *, const double *,
-                                 std::vector<PassSpec> &);
-template int final_pass<double>(Context &, const SolveGeom &, const double *, const double *, double *, std::vector<PassSpec> &);
-template int final_pass<float>(Context &, const SolveGeom &, const float *, const double *, double *, std::vector<PassSpec> &);
+                                 std::vector<PassSpec> &, const SolveTables *);
+template int final_pass<double>(Context &, const SolveGeom &, const double *, const double *, double *, std::vector<PassSpec> &,
+                                const SolveTables *);
+template int final_pass<float>(Context &, const SolveGeom &, const float *, const double *, double *, std::vector<PassSpec> &,
+                               const SolveTables *);
 
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st)
 {
@@ -366,9 +370,10 @@ int helmholtz_fp64(const double dom[6], const int N[3], const int bc[3], const d
     if (rc) return rc;
     g.stencil = stencil;
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
+    WsLease ws(ctx, st);
     const size_t bytes = sizeof(double) * (size_t)ws_row(N[0], sizeof(double)) * (N[1] + 2) * (N[2] + 2);
-    double *ex = reinterpret_cast<double *>(ctx.workspace(bytes, 0));
+    double *ex = reinterpret_cast<double *>(ws.get(bytes, 0));
     if (!ex) return fail(ERR_ALLOC, "workspace allocation failed");
     std::vector<PassSpec> v;
     if ((rc = solve_passes<double>(ctx, g, f, ex, nullptr, nullptr, v)) || (rc = final_pass<double>(ctx, g, ex, nullptr, f, v)))
@@ -392,11 +397,12 @@ int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const 
     int rc = make_geom(g, dom, N, bc, lambda, faces);
     if (rc) return rc;
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> lock(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
+    WsLease ws(ctx, st);
     const size_t wbytes = sizeof(float) * (size_t)ws_row(N[0], sizeof(float)) * (N[1] + 2) * (N[2] + 2);
     const size_t ubytes = sizeof(double) * (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
-    float *ex = reinterpret_cast<float *>(ctx.workspace(wbytes, 1));
-    double *U = refine > 0 ? reinterpret_cast<double *>(ctx.workspace(ubytes, 2)) : nullptr;
+    float *ex = reinterpret_cast<float *>(ws.get(wbytes, 1));
+    double *U = refine > 0 ? reinterpret_cast<double *>(ws.get(ubytes, 2)) : nullptr;
     if (!ex || (refine > 0 && !U)) return fail(ERR_ALLOC, "workspace allocation failed");
     PassTimer timer(ctx, st);
     // singular case: PERTRB (weighted sum of the folded rhs) in FP64, once
@@ -404,7 +410,7 @@ int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const 
     SolveArgs sa0;
     fill_solve_args(sa0, g, f, nullptr);
     if (g.singular && refine > 0) {
-        acc = reinterpret_cast<double *>(ctx.workspace(256, 3));
+        acc = reinterpret_cast<double *>(ws.get(256, 3));
         if (!acc) return fail(ERR_ALLOC, "workspace allocation failed");
         if (ctx.timing()) ctx.timing_mark("pertrb", st);
         hipError_t e = launch_fold_sum(sa0, acc + 1, st);

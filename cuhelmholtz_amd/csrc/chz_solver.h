@@ -22,6 +22,8 @@ struct SolveGeom {
 long long ws_row(int NX, size_t esz);
 int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3], double lambda,
               const double *const faces[6]);
+// slot eigenvalue tables x | y | z of a geometry (host FP64) and their cached device copy
+std::vector<double> eig_tables(const SolveGeom &g);
 const double *solver_eig(Context &ctx, const SolveGeom &g);
 void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const double *eig);
 
@@ -35,13 +37,22 @@ struct PassSpec {
     BatchDesc d;
 };
 
+// Device tables a solve's passes read: the twiddles of the x, y, z cell
+// counts (TwOffsets layout, precision of the passes) and the slot eigenvalues
+// x | y | z.  The builders take the context's cached tables when none are given.
+struct SolveTables {
+    const void *tw[3];
+    const double *eig;
+};
+
 // passes 1-4 of a solve into workspace ex (rhs read from f; U: residual of U instead)
 template <typename T>
 int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
-                 std::vector<PassSpec> &out);
+                 std::vector<PassSpec> &out, const SolveTables *tabs = nullptr);
 // inverse x pass into out (FP64: the solution; FP32: out = Uin + delta)
 template <typename T>
-int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v);
+int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v,
+               const SolveTables *tabs = nullptr);
 int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st);
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 

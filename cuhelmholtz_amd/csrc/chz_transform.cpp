@@ -99,11 +99,11 @@ int transform3d(const T *src, T *dst, int NX, int NY, int NZ, int kx, int ky, in
     if ((rc = check_axis(NX, kx, "x")) || (rc = check_axis(NY, ky, "y")) || (rc = check_axis(NZ, kz, "z"))) return rc;
     if (!src || !dst) return fail(ERR_INVALID, "null device pointer");
     Context &ctx = Context::get();
-    std::lock_guard<std::mutex> g(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> g(ctx.mutex());
     AxisCfg cx, cy, cz;
     if (!ctx.config(NX, prec, ROLE_ROWS, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) ||
         !ctx.config(NZ, prec, ROLE_ZCOLS, &cz))
-        return fail(ERR_UNSUPPORTED, "unsupported cell count (N/2 must factor into 2s, 3s and 5s)");
+        return fail(ERR_UNSUPPORTED, "no launch configuration for this cell count (LDS or thread limits)");
     const void *twx = ctx.twiddles(NX, prec), *twy = ctx.twiddles(NY, prec), *twz = ctx.twiddles(NZ, prec);
     if (!twx || !twy || !twz) return fail(ERR_ALLOC, "twiddle table allocation failed");
 
@@ -203,7 +203,7 @@ void chz_release(void)
 int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines, unsigned long long *plan)
 {
     chz::Context &ctx = chz::Context::get();
-    std::lock_guard<std::mutex> g(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> g(ctx.mutex());
     chz::AxisCfg c;
     if (role < 0 || role >= chz::ROLE_COUNT) return chz::fail(chz::ERR_INVALID, "invalid role");
     if (!ctx.config(N, prec, role, &c)) return chz::fail(chz::ERR_UNSUPPORTED, "unsupported cell count");
@@ -217,7 +217,7 @@ int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines, u
 int chz_set_axis_config(int N, int prec, int role, int ept, int tpl, int lines)
 {
     chz::Context &ctx = chz::Context::get();
-    std::lock_guard<std::mutex> g(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> g(ctx.mutex());
     if (!ctx.set_config(N, prec, role, ept, tpl, lines, nullptr))
         return chz::fail(chz::ERR_INVALID, "launch configuration cannot run (radix plan, threads or LDS)");
     return chz::OK;
@@ -226,14 +226,14 @@ int chz_set_axis_config(int N, int prec, int role, int ept, int tpl, int lines)
 void chz_reset_axis_configs(void)
 {
     chz::Context &ctx = chz::Context::get();
-    std::lock_guard<std::mutex> g(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> g(ctx.mutex());
     ctx.reset_configs();
 }
 
 int chz_load_tuning(const char *path)
 {
     chz::Context &ctx = chz::Context::get();
-    std::lock_guard<std::mutex> g(ctx.mutex());
+    std::lock_guard<std::recursive_mutex> g(ctx.mutex());
     int n = ctx.load_tuning(path);
     if (n < 0) return chz::fail(chz::ERR_INVALID, std::string("cannot read tuning table ") + path);
     return n;

@@ -5,11 +5,22 @@
  * (a hipStream_t passed as void*, NULL = the default stream) and return
  * before the device work completes; synchronise the stream to read results.
  *
+ * Streams: the single-problem calls share persistent per-device workspaces;
+ * each call's launches are ordered after the previous call that used the
+ * same workspace, on whatever stream (a stream wait on that call's end event,
+ * no host wait), so calls on different streams serialise on the device where
+ * they share a workspace and never overwrite each other's data.  A batched
+ * plan is ordered the same way after its own previous solve or rebind.
+ *
  * Status: 0 = OK, -1 = invalid argument, -2 = unsupported size,
  * -3 = device / launch error, -4 = device allocation failure.
  * chz_last_error() returns the message of the last failure on this thread.
  *
- * Supported sizes: even N with N/2 = 2^a 3^b 5^c (up to 4096 per axis).
+ * Supported sizes: any even N up to 4096 per axis (like the reference's cuFFT
+ * plans, cudasymmfft/funcinterface.cu:54-60).  The FFT stages use radix 2, 3,
+ * 4, 5, 6, 8, 10, 12 and 16 butterflies; other prime factors of N/2 run as
+ * direct DFT stages (M * p multiply-adds per line for a factor p), which are
+ * correct for every size but slow for large primes.
  */
 #ifndef CUHELMHOLTZ_H
 #define CUHELMHOLTZ_H
@@ -131,8 +142,9 @@ typedef struct chz_problem {
  * boundary codes): precision 64 = plain FP64 (refine must be 0), precision
  * 32 = FP32 transforms + `refine` FP64 refinement steps as in
  * chz_helmholtz_solve_mixed (singular problems then need refine == 0).  The
- * plan owns one workspace per problem and device tables of per-problem pass
- * descriptors, and holds the whole solve captured as one hipGraph: one
+ * plan owns one workspace per problem, device tables of per-problem pass
+ * descriptors and its own copies of the twiddle and eigenvalue tables it
+ * reads, and holds the whole solve captured as one hipGraph: one
  * persistent launch per pass and size class, then the periodic copies.  The
  * problems' f arrays and face arrays are bound here: each chz_batch_solve
  * solves whatever they hold at that time.  Results equal the single-problem
@@ -147,10 +159,11 @@ int chz_batch_solve(void *plan, int use_graph, void *stream);
 
 /* Point a plan at `count` problems of the same geometry as the ones it was
  * made for (sizes, codes, domains, which face arrays are given, lambda = 0 or
- * not): new f and face arrays and new lambdas.  Waits for the work queued on
- * `stream`, then rewrites and uploads the pass descriptors on it; the plan's
- * workspaces, work lists and graph are kept.  Calls that overlap with a
- * solve of the same plan on another stream race.  No reference counterpart. */
+ * not): new f and face arrays and new lambdas.  The pass descriptors are
+ * rewritten on the host and uploaded on `stream` (from a pinned staging
+ * buffer), ordered after the plan's earlier solves on any stream; the plan's
+ * workspaces, work lists, tables and graph are kept.  The host waits only for
+ * the plan's previous rebind upload to have run.  No reference counterpart. */
 int chz_batch_rebind(void *plan, const chz_problem *problems, int count, void *stream);
 
 /* Batched solve through a plan kept by the library for repeated solves of
@@ -164,7 +177,7 @@ int chz_batch_solve_cached(const chz_problem *problems, int count, int precision
 /* Launch count of one batched solve and the plan's device memory. */
 int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *table_bytes);
 
-/* Free the plan (synchronise any stream running it first). */
+/* Free the plan (waits for the plan's last solve or rebind to finish). */
 int chz_batch_destroy(void *plan);
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -172,6 +185,8 @@ const char *chz_last_error(void);
 
 /* Free cached twiddle tables and workspaces of the current device, and the
  * batched plans kept by chz_batch_solve_cached (after waiting for the device).
+ * Plans made by chz_batch_create own their tables and stay usable; launch
+ * configurations (tuning) are kept.
  * Replaces freeMemory_cubic (reference cudasymmfft/funcinterface.cu:337-342). */
 void chz_release(void);
 

@@ -12,9 +12,9 @@
  * 2 Dirichlet-Neumann, 3 Neumann-Neumann, 4 Neumann-Dirichlet.
  * IERROR: 0 ok; 1 XS >= XF; 2 L < 3; 3 LBDCND outside 0..4; 4 YS >= YF;
  * 5 M < 3; 6 MBDCND; 7 ZS >= ZF; 8 N < 3; 9 NBDCND; 10 LDIMF < L+1;
- * 11 MDIMF < M+1; 12 ELMBDA > 0 (warning, solution computed); 13 cell count
- * not supported by the GPU transforms (odd, or N/2 not 2^a 3^b 5^c: FISHPACK
- * accepts any size); 20 device failure (chz_last_error() has the reason).
+ * 11 MDIMF < M+1; 12 ELMBDA > 0 (warning, solution computed); 13 odd cell
+ * count (the GPU transforms take any even one; FISHPACK accepts odd ones
+ * too); 20 device failure (chz_last_error() has the reason).
  * PERTRB: for ELMBDA = 0 with periodic / Neumann codes only, the weighted mean
  * of the folded rhs (the constant removed to make the problem consistent);
  * the solution returned then has a zero (0,0,0) mode.  W is not used.

@@ -433,6 +433,9 @@ static int helmholtz_impl(double left, double right, int NX, int xbc, double *bc
     }
     double ih[3] = {NX / (right - left), NY / (top - bottom), NZ / (back - front)};  /* :44-46 */
     double *bl[3] = {bcl, bcb, bcf}, *br[3] = {bcr, bct, bce};
+    /* Neumann ends read their face array (the reference dereferences it unchecked) */
+    for (int a = 0; a < 3; a++)
+        if (((bc[a] == 3 || bc[a] == 4) && !bl[a]) || ((bc[a] == 2 || bc[a] == 3) && !br[a])) return -1;
     bc_fold(f, Nn, bc, s, e, ih, bl, br);
 
     int64_t px = NX + 2, py = NY + 2, pz = NZ + 2;

@@ -72,8 +72,11 @@ def test_batch_fp64_matches_single_small():
     bs.close()
 
 
-def test_batch_fp64_vs_oracle_sample():
-    probs = make(small_descs(20, 12))
+@pytest.mark.parametrize("sizes", [(8, 12, 16, 20, 24), (14, 22, 26, 34, 98)])
+def test_batch_fp64_vs_oracle_sample(sizes):
+    """A sample against the oracle; the second size set has N/2 with prime
+    factors 7, 11, 13, 17 (the batched passes on direct FFT stages)."""
+    probs = make(small_descs(20, 12, sizes))
     bs = run_batch(probs)
     for p in probs[::3]:
         d = p["d"]
@@ -267,3 +270,31 @@ def test_release_then_solve_again():
     C.helmholtz_solve(w, n, bc, -1.5, [face if b is not None else None for b in bcs])
     torch.cuda.synchronize()
     assert rel(u2, u1) <= SAME and rel(u2[1], w) <= SAME
+
+
+def test_batch_allocation_failure_reported():
+    """A plan whose workspaces exceed the device memory (40 problems of
+    1024^3 cells, 350 GB) fails with the allocation status and message, and
+    the library keeps working afterwards."""
+    import ctypes
+    f = torch.zeros(8, dtype=torch.float64, device=dev())   # never read: the workspace allocation comes first
+    arr = (C._Problem * 40)()
+    for q in arr:
+        for i in range(6):
+            q.domain[i] = float(i % 2)
+            q.faces[i] = None
+        for a in range(3):
+            q.N[a] = 1024
+            q.bc[a] = 1
+        q.lam = -1.0
+        q.f = f.data_ptr()
+    plan = ctypes.c_void_p()
+    rc = C.lib().chz_batch_create(arr, 40, 64, 0, ctypes.byref(plan))
+    assert rc == -4 and not plan.value
+    assert b"allocation failed" in C.lib().chz_last_error()
+    p = make(small_descs(3, 14))[0]
+    u = single(p)
+    torch.cuda.synchronize()
+    d = p["d"]
+    want = O.helmholtz(p["f_np"], d["n"], d["bc"], d["lam"], p["bcs_np"])
+    assert float(np.max(np.abs(u.cpu().numpy() - want)) / np.max(np.abs(want))) <= 1e-12

@@ -62,8 +62,11 @@ def test_golden_sparse_direct():
 
 
 @pytest.mark.parametrize("n,dom", [((6, 8, 10), (0.0, 1.0) * 3), ((12, 4, 6), (0.0, 2.0, -1.0, 0.5, 0.25, 1.0)),
-                                   ((16, 24, 8), (0.0, 1.0) * 3)])
+                                   ((16, 24, 8), (0.0, 1.0) * 3), ((14, 22, 26), (0.0, 1.0) * 3),
+                                   ((34, 28, 42), (0.0, 1.5, 0.0, 1.0, -1.0, 1.0))])
 def test_all_bc_vs_oracle(n, dom):
+    """Every code triple against the oracle, including sizes whose N/2 has
+    prime factors 7, 11, 13, 17 (direct FFT stages)."""
     rng = np.random.default_rng(1)
     for bc in itertools.product(range(5), repeat=3):
         f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
@@ -209,7 +212,7 @@ def mixed_solve(f, n, bc, lam, bcs, refine, dom=(0.0, 1.0) * 3):
     return ft.cpu().numpy()
 
 
-@pytest.mark.parametrize("n", [(6, 8, 10), (16, 12, 8)])
+@pytest.mark.parametrize("n", [(6, 8, 10), (16, 12, 8), (14, 22, 26)])
 def test_mixed_all_bc_converges(n):
     """Error vs the FP64 solve: FP32 alone ~1e-6, one refinement < 1e-10, two < 1e-12."""
     rng = np.random.default_rng(11)

@@ -79,9 +79,13 @@ def test_all_kind_triples_small(n):
     print("worst rel err", worst)
 
 
-@pytest.mark.parametrize("N", [2, 4, 6, 8, 12, 16, 18, 24, 32, 36, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024])
+@pytest.mark.parametrize("N", [2, 4, 6, 8, 12, 16, 18, 24, 32, 36, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024,
+                               14, 22, 26, 28, 34, 42, 98, 700, 442, 514, 1018])
 def test_axis_lengths_all_kinds(N):
-    """Every kind along every axis for the supported lengths (all radix plans)."""
+    """Every kind along every axis for 2,3,5-smooth lengths (all specialised
+    radix plans) and for lengths whose N/2 has other prime factors (direct
+    stages: 7, 11, 13, 17, 7^2, 2 5^2 7, 13 17, 257 and 509 -- the last two
+    larger than a plan byte, run as one direct stage of the remaining factor)."""
     rng = np.random.default_rng(N)
     for axis in range(3):
         for kind in range(8):
@@ -186,8 +190,9 @@ def test_errors_are_reported():
     y = torch.zeros((8, 8, 8), dtype=torch.float64, device=dev())
     with pytest.raises(C.CHZError, match="kind"):
         C.do_transform(y, None, (9, O.DCT_2, O.DCT_2))
-    z = torch.zeros((8, 8, 16), dtype=torch.float64, device=dev())  # NX = 14: N/2 = 7 unsupported
-    with pytest.raises(C.CHZError, match="unsupported"):
+    # NX = 16400: more points per line than a workgroup holds (512 threads x 16)
+    z = torch.zeros((4, 4, 16402), dtype=torch.float64, device=dev())
+    with pytest.raises(C.CHZError, match="no launch configuration"):
         C.do_transform(z, None, (O.DCT_2, O.DCT_2, O.DCT_2))
 
 
@@ -206,5 +211,34 @@ def test_cpp_dropin_do_transform():
     y = torch.empty_like(x)
     torch.cuda.synchronize()
     fn(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), *n, *kinds)
+    sl = out_slices(kinds, n)
+    assert rel(y.cpu().numpy()[sl], ref[sl]) < TOL
+
+
+def test_cpp_dropin_cubic_and_nocubic():
+    """do_transform_cubic (in place) and do_transform_nocubic (result in d_out),
+    reference cudasymmfft/funcinterface.h:14-29, against the oracle."""
+    import ctypes
+    L = C.lib()
+    cubic = getattr(L, "_Z18do_transform_cubicPdiii14transform_kindS0_S0_")
+    cubic.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6
+    nocubic = getattr(L, "_Z20do_transform_nocubicPdS_iii14transform_kindS0_S0_")
+    nocubic.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
+    rng = np.random.default_rng(6)
+    for n, kinds in [((16, 16, 16), (O.DCT_2, O.DST_1, O.DFT_R2C)), ((14, 14, 14), (O.DST_2, O.DCT_3, O.DCT_1))]:
+        v = rand_volume(rng, kinds, n)
+        ref = O.transform(v, *kinds)
+        sl = out_slices(kinds, n)
+        x = torch.from_numpy(v).to(dev())
+        torch.cuda.synchronize()
+        cubic(ctypes.c_void_p(x.data_ptr()), *n, *kinds)
+        assert rel(x.cpu().numpy()[sl], ref[sl]) < TOL, (n, "cubic")
+    n, kinds = (16, 12, 8), (O.DST_3, O.DCT_1, O.DFT_R2C)
+    v = rand_volume(rng, kinds, n)
+    ref = O.transform(v, *kinds)
+    x = torch.from_numpy(v).to(dev())
+    y = torch.full_like(x, float("nan"))
+    torch.cuda.synchronize()
+    nocubic(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), *n, *kinds)
     sl = out_slices(kinds, n)
     assert rel(y.cpu().numpy()[sl], ref[sl]) < TOL

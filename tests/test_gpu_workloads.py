@@ -1,0 +1,246 @@
+"""GPU parity at the benchmark workloads' own sizes, through the C ABI.
+
+* The 2048-problem batched workload (sizes 32..128 per axis, codes 0-4,
+  numpy default_rng(2)): a plan over a stratified subset (every size on every
+  axis, every code on every axis) against the oracle's restatement of
+  cuda_helmholtz_solver (cudahelmholtz/cuda_helmholtz_solver.cu:33-380) at
+  1e-12, FP32 + refinement against FP64 at 1e-10; the full 2048-problem FP64
+  plan through the residual invariant on every problem.
+* Single solves at N = 48 and 96 in the first-pass (x) and fused-z roles
+  against the oracle.
+* do_transform (cudasymmfft/funcinterface.cu:25-334) for all 343 forward kind
+  triples at 128^3, 256^3, 512^3 (in place and out of place) and 512x384x256
+  (out of place) through separable inputs, every triple at 128^3 on random
+  inputs against the oracle's 3D transform, a sample at 256^3.
+* The 512^3 codes (3,4,0) solve against the hw3crt-contract CPU solver at
+  1e-12 for lambda in {-4, 0.5}.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not torch.cuda.is_available(), reason="no GPU")]
+
+import cuhelmholtz_amd as C  # noqa: E402
+from cuhelmholtz_amd import checks as H  # noqa: E402
+from cuhelmholtz_amd import workloads as W  # noqa: E402
+
+TOL = 1e-12
+FWD7 = [O.DFT_R2C, O.DCT_1, O.DCT_2, O.DCT_3, O.DST_1, O.DST_2, O.DST_3]
+SIZES = (32, 48, 64, 96, 128)
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def relnp(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def reltorch(a, b):
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+
+def stratified(descs):
+    """Indices covering every size on every axis and every code on every axis
+    (greedy, in workload order), topped up to 30 problems."""
+    need = {("n", a, v) for a in range(3) for v in SIZES} | {("bc", a, v) for a in range(3) for v in range(5)}
+    pick = []
+    for i, d in enumerate(descs):
+        keys = {("n", a, d["n"][a]) for a in range(3)} | {("bc", a, d["bc"][a]) for a in range(3)}
+        if keys & need:
+            pick.append(i)
+            need -= keys
+        if not need:
+            break
+    assert not need, need
+    for i in range(len(descs)):
+        if len(pick) >= 30:
+            break
+        if i not in pick:
+            pick.append(i)
+    return sorted(pick)
+
+
+def test_batched_workload_stratified_vs_oracle():
+    descs = W.batched_problems(count=2048, seed=2)
+    sub = [descs[i] for i in stratified(descs)]
+    arrays = [W.problem_arrays(d) for d in sub]
+    want = [O.helmholtz(f, d["n"], d["bc"], d["lam"], bcs) for d, (f, bcs) in zip(sub, arrays)]
+    f64, f32, probs64, probs32 = [], [], [], []
+    for d, (f, bcs) in zip(sub, arrays):
+        bt = [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs]
+        for lst, pl in ((f64, probs64), (f32, probs32)):
+            t = torch.from_numpy(f).to(dev())
+            lst.append(t)
+            pl.append((t, d["n"], d["bc"], d["lam"], bt))
+    b64 = C.BatchSolver(probs64, 64)
+    b32 = C.BatchSolver(probs32, 32, refine=2)
+    b64.solve()
+    b32.solve()
+    torch.cuda.synchronize()
+    worst64 = worst32 = 0.0
+    for k, d in enumerate(sub):
+        e64 = relnp(f64[k].cpu().numpy(), want[k])
+        e32 = reltorch(f32[k], f64[k])
+        worst64, worst32 = max(worst64, e64), max(worst32, e32)
+        assert e64 <= TOL, (d, e64)
+        assert e32 <= 1e-10, (d, e32)
+    print(f"{len(sub)} problems: FP64 vs oracle {worst64:.2e}, FP32+2 vs FP64 {worst32:.2e}")
+    b64.close()
+    b32.close()
+
+
+def test_batched_workload_full_residual():
+    """All 2048 problems in one FP64 plan (f and face data drawn on the
+    device from each problem's seed): the normalised 7-point residual of every
+    solution."""
+    descs = W.batched_problems(count=2048, seed=2)
+    probs, f0 = [], []
+    for d in descs:
+        NX, NY, NZ = d["n"]
+        g = torch.Generator(device=dev())
+        g.manual_seed(d["seed"])
+        f = torch.randn((NZ + 1, NY + 1, NX + 1), generator=g, dtype=torch.float64, device=dev())
+        sizes = [(NZ + 1) * (NY + 1)] * 2 + [(NZ + 1) * (NX + 1)] * 2 + [(NY + 1) * (NX + 1)] * 2
+        bcs = [torch.randn(sizes[i], generator=g, dtype=torch.float64, device=dev())
+               if d["bc"][i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None for i in range(6)]
+        f0.append(f.clone())
+        probs.append((f, d["n"], d["bc"], d["lam"], bcs))
+    bs = C.BatchSolver(probs, 64)
+    bs.solve()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k, (d, p) in enumerate(zip(descs, probs)):
+        r = H.relative_residual(p[0], f0[k], d["n"], d["bc"], d["lam"], p[4])
+        worst = max(worst, r)
+        assert r < 1e-13, (k, d, r)
+    print(f"2048 problems: worst normalised residual {worst:.2e}")
+    bs.close()
+
+
+@pytest.mark.parametrize("n,bc", [((48, 40, 96), (3, 4, 0)), ((96, 32, 48), (2, 0, 4)), ((48, 96, 48), (0, 1, 3)),
+                                  ((96, 48, 96), (4, 3, 2))])
+def test_single_solve_48_96_vs_oracle(n, bc):
+    """N = 48 and 96 in the first-pass (x) and fused-z roles, FP64 against the
+    oracle and FP32 + 1 refinement against FP64."""
+    rng = np.random.default_rng(sum(n) + 10 * sum(bc))
+    f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+    sizes = [(n[2] + 1) * (n[1] + 1)] * 2 + [(n[2] + 1) * (n[0] + 1)] * 2 + [(n[1] + 1) * (n[0] + 1)] * 2
+    bcs = [rng.standard_normal(sizes[i]) if bc[i // 2] in ((3, 4) if i % 2 == 0 else (2, 3)) else None
+           for i in range(6)]
+    for lam in (-3.0, 0.5):
+        want = O.helmholtz(f, n, bc, lam, bcs)
+        bt = [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs]
+        u = torch.from_numpy(f).to(dev())
+        C.helmholtz_solve(u, n, bc, lam, bt)
+        m = torch.from_numpy(f).to(dev())
+        C.helmholtz_solve_mixed(m, n, bc, lam, bt, refine=1)
+        torch.cuda.synchronize()
+        assert relnp(u.cpu().numpy(), want) <= TOL, lam
+        assert reltorch(m, u) <= 1e-10, lam
+
+
+def line_of(rng, kind, N):
+    x = np.zeros(N + 2)
+    lo, hi = O.in_window(kind, N)
+    x[lo:hi + 1] = rng.uniform(-1, 1, hi - lo + 1)
+    return x
+
+
+def out_slices(kinds, n):
+    w = [O.out_window(k, N) for k, N in zip(kinds, n)]
+    return (slice(w[2][0], w[2][1] + 1), slice(w[1][0], w[1][1] + 1), slice(w[0][0], w[0][1] + 1))
+
+
+@pytest.mark.parametrize("n,modes", [((128, 128, 128), (True, False)), ((256, 256, 256), (True, False)),
+                                     ((512, 512, 512), (True, False)), ((512, 384, 256), (False,))])
+def test_all_343_triples_separable(n, modes):
+    """Rank-1 inputs a_i b_j c_k: the transform is the outer product of the
+    three 1-D oracle rows, for every forward kind triple; in place (cubes) and
+    out of place into a buffer pre-filled with NaN."""
+    rng = np.random.default_rng(5)
+    NX, NY, NZ = n
+    lines = {(k, N): line_of(rng, k, N) for k in FWD7 for N in set(n)}
+    rows = {key: torch.from_numpy(O.line(key[0], v)).to(dev()) for key, v in lines.items()}
+    ins = {key: torch.from_numpy(v).to(dev()) for key, v in lines.items()}
+    x = torch.empty((NZ + 2, NY + 2, NX + 2), dtype=torch.float64, device=dev())
+    y = torch.empty_like(x)
+    worst = 0.0
+    for kinds in itertools.product(FWD7, repeat=3):
+        a, b, c = (ins[(k, N)] for k, N in zip(kinds, n))
+        ta, tb, tc = (rows[(k, N)] for k, N in zip(kinds, n))
+        sl = out_slices(kinds, n)
+        ref = (tc[:, None, None] * tb[None, :, None] * ta[None, None, :])[sl]
+        for inplace in modes:
+            torch.mul(c[:, None, None] * b[None, :, None], a[None, None, :], out=x)
+            if inplace:
+                out = C.do_transform(x, None, kinds)
+            else:
+                y.fill_(float("nan"))
+                out = C.do_transform(x, y, kinds)
+            err = ((out[sl] - ref).abs().max() / ref.abs().max()).item()
+            worst = max(worst, err)
+            assert err < TOL, (kinds, inplace, err)
+    print(f"{n}: worst {worst:.2e}")
+    del x, y
+    torch.cuda.empty_cache()
+
+
+def rand_volume(rng, kinds, n):
+    NX, NY, NZ = n
+    v = np.zeros((NZ + 2, NY + 2, NX + 2))
+    w = [O.in_window(k, N) for k, N in zip(kinds, n)]
+    sl = (slice(w[2][0], w[2][1] + 1), slice(w[1][0], w[1][1] + 1), slice(w[0][0], w[0][1] + 1))
+    v[sl] = rng.uniform(-1, 1, v[sl].shape)
+    return v
+
+
+@pytest.mark.parametrize("n,count", [((128, 128, 128), 343), ((256, 256, 256), 8)])
+def test_random_triples_vs_oracle(n, count):
+    """Random inputs (uniform[-1,1], seed 0, input windows only) against the
+    oracle's 3D transform: every forward triple at 128^3, a sample at 256^3;
+    alternately in place and out of place."""
+    rng = np.random.default_rng(0)
+    triples = list(itertools.product(FWD7, repeat=3))
+    if count < len(triples):
+        triples = [triples[i] for i in rng.choice(len(triples), count, replace=False)]
+    worst = 0.0
+    for t, kinds in enumerate(triples):
+        v = rand_volume(rng, kinds, n)
+        ref = O.transform(v, *kinds)
+        x = torch.from_numpy(v).to(dev())
+        out = C.do_transform(x, None if t % 2 == 0 else torch.empty_like(x), kinds)
+        sl = out_slices(kinds, n)
+        err = relnp(out.cpu().numpy()[sl], ref[sl])
+        worst = max(worst, err)
+        assert err < TOL, (kinds, err)
+        if t % 49 == 48:
+            print(f"{n}: {t + 1} triples checked", flush=True)
+    print(f"{n}: {len(triples)} triples, worst {worst:.2e}")
+
+
+@pytest.mark.parametrize("lam", [-4.0, 0.5])
+def test_512_codes_340_vs_hw3crt(lam):
+    """The mixed-precision workload at 512^3, codes (3,4,0): the FP64 GPU solve
+    against the hw3crt-contract CPU solver (x / y transforms, z tridiagonal
+    solves) at 1e-12 relative L-infinity."""
+    n, bc = (512, 512, 512), (3, 4, 0)
+    f, bcs = W.smooth_modes(n)
+    want, ierr, _ = O.solver_3d(f, n, bc, lam, bcs, threads=16)
+    assert ierr == (12 if lam > 0 else 0)   # 12: FISHPACK's lambda > 0 warning, solution computed
+    u = torch.from_numpy(f).to(dev())
+    del f
+    C.helmholtz_solve(u, n, bc, lam, [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs])
+    torch.cuda.synchronize()
+    got = u.cpu().numpy()
+    del u
+    torch.cuda.empty_cache()
+    e = relnp(got, want)
+    print(f"512^3 lambda {lam}: {e:.2e} against hw3crt")
+    assert e <= TOL, e

@@ -76,11 +76,12 @@ def test_ierror_matches_oracle(change, code):
 
 
 def test_unsupported_size_reported():
-    """N/2 must be 2,3,5-smooth for the GPU transforms (FISHPACK takes any size)."""
-    f = np.zeros(23 * 9 * 9)
-    _, ierr = C.hw3crt(f, (22, 8, 8), (1, 1, 1), -1.0)
+    """Odd cell counts are rejected (the GPU transforms take any even count;
+    FISHPACK takes odd ones too)."""
+    f = np.zeros(22 * 9 * 9)
+    _, ierr = C.hw3crt(f, (21, 8, 8), (1, 1, 1), -1.0)
     assert ierr == 13
-    assert b"2^a 3^b 5^c" in C.lib().chz_last_error()
+    assert b"must be even" in C.lib().chz_last_error()
 
 
 def test_drivers_built_and_print_usage():
