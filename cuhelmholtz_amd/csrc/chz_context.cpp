@@ -19,6 +19,10 @@ void set_error(const std::string &msg) { g_err = msg; }
 int fail(int code, const std::string &msg)
 {
     g_err = msg;
+    // a failed HIP call (out of memory, launch error) also leaves HIP's
+    // per-thread last error set: clear it, the status carries the failure
+    // (otherwise the caller's next HIP user, e.g. torch, reports it again)
+    if (code == ERR_ALLOC || code == ERR_DEVICE) (void)hipGetLastError();
     return code;
 }
 const char *last_error() { return g_err.c_str(); }

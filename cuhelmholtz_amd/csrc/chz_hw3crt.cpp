@@ -100,7 +100,7 @@ static int hw3crt_gpu(double xs, double xf, int l, int lbdcnd, const double *bdx
         if (!hface[i].empty()) dface[i] = d.get(hface[i].size());
     double *dacc = d.get(4);
     if (!df || !dacc) {
-        set_error("hw3crt_: device allocation failed");
+        fail(ERR_ALLOC, "hw3crt_: device allocation failed");
         return 20;
     }
     hipStream_t st = nullptr;
@@ -109,7 +109,7 @@ static int hw3crt_gpu(double xs, double xf, int l, int lbdcnd, const double *bdx
         if (ok && dface[i])
             ok = hipMemcpy(dface[i], hface[i].data(), sizeof(double) * hface[i].size(), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
-        set_error("hw3crt_: host to device copy failed");
+        fail(ERR_DEVICE, "hw3crt_: host to device copy failed");
         return 20;
     }
 
@@ -124,7 +124,7 @@ static int hw3crt_gpu(double xs, double xf, int l, int lbdcnd, const double *bdx
         fill_solve_args(sa, g, df, nullptr);
         double acc = 0.0;
         if (launch_fold_sum(sa, dacc, st) != hipSuccess || hipMemcpy(&acc, dacc, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
-            set_error("hw3crt_: PERTRB reduction failed");
+            fail(ERR_DEVICE, "hw3crt_: PERTRB reduction failed");
             return 20;
         }
         *pertrb = acc * mode0_inv_wsum(sa);
