@@ -62,6 +62,8 @@ def _declare_solver(L):
         L.chz_helmholtz_solve.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve19"):
         L.chz_helmholtz_solve19.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
+    if hasattr(L, "chz_helmholtz_solve_noalloc"):
+        L.chz_helmholtz_solve_noalloc.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, vp]
     if hasattr(L, "chz_helmholtz_solve_mixed"):
         L.chz_helmholtz_solve_mixed.argtypes = ([d, d, i, i, vp, vp] * 3) + [d, vp, i, vp]
     if hasattr(L, "chz_helmholtz_adjoint"):

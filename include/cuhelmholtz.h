@@ -68,6 +68,28 @@ int chz_helmholtz_solve(double left, double right, int NX, int xbc, const double
                         double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
                         double lambda, double *f, void *stream);
 
+/* Scratch of chz_helmholtz_solve_noalloc: a fixed device array of this many
+ * bytes, held by every device the library runs on until the process ends. */
+#define CHZ_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* chz_helmholtz_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and result (to rounding: its twiddle and eigenvalue
+ * tables are generated on the device) as chz_helmholtz_solve; the spectral
+ * workspace and the tables live in a static device array of
+ * CHZ_NOALLOC_SCRATCH_BYTES, so the call only launches kernels (no
+ * allocation, copy or memset).  Needs
+ *   8 (NX+2 rounded up to 16)(NY+2)(NZ+2) + 16 (3 N_a + 2) per axis + 8 (NX+NY+NZ+6)
+ * bytes (each part rounded up to 256) within that size -- about 190^3 cells;
+ * larger problems return -1 with a message.  Every call on one device shares
+ * the array and leaves its contents behind: calls that overlap on one device
+ * (two streams, two threads) race. */
+int chz_helmholtz_solve_noalloc(double left, double right, int NX, int xbc, const double *bcl, const double *bcr,
+                                double bottom, double top, int NY, int ybc, const double *bcb, const double *bct,
+                                double front, double back, int NZ, int zbc, const double *bcf, const double *bce,
+                                double lambda, double *f, void *stream);
+
 /* The same solve with the 19-point compact operator: (L19 + lambda) u = B f,
  * L19 = sum_a D_a + sum_{a<b} (h_a^2 + h_b^2)/12 D_a D_b and B = I + sum_a
  * h_a^2/12 D_a, D_a the second difference along axis a with the boundary
