@@ -86,6 +86,8 @@ def _declare_batch(L):
     L.chz_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_longlong)]
     L.chz_batch_destroy.argtypes = [vp]
+    if hasattr(L, "chz_batch_refine_counts"):
+        L.chz_batch_refine_counts.argtypes = [vp, ctypes.POINTER(i), i]
 
 
 def header_symbols():
@@ -387,7 +389,9 @@ class BatchSolver:
     problems: sequence of (f, n, bc, lam[, bcs[, dom]]) with f a contiguous
     CUDA float64 node tensor shaped (NZ+1, NY+1, NX+1) -- bound to the plan:
     solve() solves whatever the tensors hold, in place.  prec 64 = FP64,
-    prec 32 = FP32 transforms + `refine` FP64 refinement steps.
+    prec 32 = FP32 transforms + `refine` FP64 refinement steps (refine < 0:
+    per-problem counts up to -refine from each operator's spectral
+    condition number, see refine_counts()).
     """
 
     def __init__(self, problems, prec=64, refine=0):
@@ -450,6 +454,12 @@ class BatchSolver:
             s = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
             _check(lib().chz_batch_rebind(self._plan, arr, len(problems), s))
         self._keep = keep
+
+    def refine_counts(self):
+        """Refinement steps of each problem (chz_batch_refine_counts)."""
+        arr = (ctypes.c_int * self.count)()
+        _check(lib().chz_batch_refine_counts(self._plan, arr, self.count))
+        return list(arr)
 
     def info(self):
         n, w, t = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()

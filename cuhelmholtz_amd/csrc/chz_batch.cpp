@@ -64,7 +64,8 @@ struct GroupRec {
 };
 
 struct BatchPlan {
-    int dev = 0, count = 0, prec = 64, refine = 0;
+    int dev = 0, count = 0, prec = 64, refine = 0;   // refine < 0: per-problem counts up to -refine
+    std::vector<int> kref;                            // refinement steps of each problem
     void *work = nullptr;     // per-problem workspaces
     void *tables = nullptr;   // device image: descriptors, periodic copies | items | twiddles, eigenvalues
     size_t work_bytes = 0, table_bytes = 0;
@@ -171,9 +172,9 @@ static int build_specs(Context &ctx, const chz_problem *probs, const BatchPlan &
                        std::vector<std::vector<PassSpec>> &specs)
 {
     char *wbase = reinterpret_cast<char *>(plan.work);
-    const int refine = plan.refine;
     specs.assign(plan.count, {});
     for (int i = 0; i < plan.count; i++) {
+        const int refine = plan.kref[i];
         const SolveGeom &g = geo[i];
         double *f = probs[i].f;
         std::vector<PassSpec> &v = specs[i];
@@ -219,6 +220,20 @@ static void write_descs(BatchPlan &plan, const chz_problem *probs, const std::ve
     }
 }
 
+// Refinement steps of one problem when the plan chooses them (refine < 0).
+// The error of the FP32 solve after k steps grows with the spectral condition
+// number kappa = max|e + lambda| / min|e + lambda|: on the 2048-problem
+// workload (sizes 32..128, codes 0-4, lambda in [-10, 0.5]) one step left
+// every problem with kappa <= 1e5 within 4e-11 of the FP64 solve, and up to
+// 1e-9 above that.  One step up to kappa = 1e5, two up to 1e7, three beyond;
+// at most kmax.
+static int auto_refine(const SolveGeom &g, int kmax)
+{
+    const double kappa = spectral_condition(g);
+    const int k = kappa <= 1e5 ? 1 : kappa <= 1e7 ? 2 : 3;
+    return std::min(k, kmax);
+}
+
 static int create(const chz_problem *probs, int count, int prec, int refine, BatchPlan **out)
 {
     if (!out) return fail(ERR_INVALID, "null plan pointer");
@@ -226,26 +241,28 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     if (!probs || count <= 0) return fail(ERR_INVALID, "empty problem list");
     if (prec != 64 && prec != 32) return fail(ERR_INVALID, "precision must be 64 or 32");
     if (prec == 64 && refine != 0) return fail(ERR_INVALID, "the FP64 batched solve takes no refinement steps");
-    if (refine < 0 || refine > 16) return fail(ERR_INVALID, "refinement count must be in 0..16");
+    if (refine < -16 || refine > 16) return fail(ERR_INVALID, "refinement count must be in -16..16");
     Context &ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
 
     std::vector<SolveGeom> geo(count);
     const size_t esz = prec == 64 ? 8 : 4;
     std::vector<size_t> ws_off(count), u_off(count, 0);
+    std::vector<int> kref(count, refine);
     size_t total = 0;
     for (int i = 0; i < count; i++) {
         const chz_problem &q = probs[i];
         if (!q.f) return fail(ERR_INVALID, "problem " + std::to_string(i) + ": null solution array");
         int rc = make_geom(geo[i], q.domain, q.N, q.bc, q.lambda, q.faces);
         if (rc) return fail(rc, "problem " + std::to_string(i) + ": " + last_error());
-        if (prec == 32 && refine > 0 && geo[i].singular)
+        if (refine < 0) kref[i] = auto_refine(geo[i], -refine);
+        if (prec == 32 && kref[i] > 0 && geo[i].singular)
             return fail(ERR_UNSUPPORTED, "problem " + std::to_string(i) +
                                              ": singular problems (lambda = 0, periodic/Neumann-Neumann on every axis) "
                                              "are solved in FP64 or alone in mixed precision");
         ws_off[i] = total;
         total += align256(esz * (size_t)ws_row(q.N[0], esz) * (q.N[1] + 2) * (q.N[2] + 2));
-        if (prec == 32 && refine > 0) {
+        if (prec == 32 && kref[i] > 0) {
             u_off[i] = total;
             total += align256(sizeof(double) * (size_t)(q.N[0] + 1) * (q.N[1] + 1) * (q.N[2] + 1));
         }
@@ -254,6 +271,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     plan->count = count;
     plan->prec = prec;
     plan->refine = refine;
+    plan->kref = kref;
     plan->dev = ctx.device();
     plan->work_bytes = total;
     plan->ws_off = ws_off;
@@ -271,12 +289,10 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         delete plan;
         return rc;
     }
-    const size_t nsteps = specs[0].size();
-    for (int i = 1; i < count; i++)
-        if (specs[i].size() != nsteps) {
-            delete plan;
-            return fail(ERR_INVALID, "internal: uneven pass sequences");
-        }
+    // problems with fewer refinement steps have shorter sequences: step s
+    // groups the problems that have one
+    size_t nsteps = 0;
+    for (int i = 0; i < count; i++) nsteps = std::max(nsteps, specs[i].size());
 
     // group step s of all problems by launch configuration
     int cus = 0;
@@ -285,6 +301,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     for (size_t s = 0; s < nsteps; s++) {
         std::map<LaunchKey, size_t> idx;
         for (int i = 0; i < count; i++) {
+            if (s >= specs[i].size()) continue;
             const PassSpec &q = specs[i][s];
             const LaunchKey key = std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
             auto it = idx.find(key);
@@ -496,6 +513,11 @@ static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStrea
         if (rc) return fail(rc, "problem " + std::to_string(i) + ": " + last_error());
         if (!(key_of(geo[i], q) == plan.keys[i]))
             return fail(ERR_INVALID, "rebind: problem " + std::to_string(i) + " differs in geometry from the plan's");
+        if (plan.refine < 0 && auto_refine(geo[i], -plan.refine) != plan.kref[i]) {
+            if (cfg_changed) *cfg_changed = true;
+            return fail(ERR_INVALID, "rebind: the new lambda of problem " + std::to_string(i) +
+                                         " changes its refinement count; make a new plan");
+        }
     }
     std::vector<std::vector<PassSpec>> specs;
     int rc = build_specs(ctx, probs, plan, geo, specs);
@@ -672,6 +694,15 @@ int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *
     if (launches) *launches = n;
     if (work_bytes) *work_bytes = (long long)p->work_bytes;
     if (table_bytes) *table_bytes = (long long)p->table_bytes;
+    return chz::OK;
+}
+
+int chz_batch_refine_counts(void *plan, int *counts, int count)
+{
+    if (!plan || !counts) return chz::fail(chz::ERR_INVALID, "null plan or counts");
+    const BatchPlan *p = reinterpret_cast<const BatchPlan *>(plan);
+    if (count != p->count) return chz::fail(chz::ERR_INVALID, "count differs from the plan's problem count");
+    for (int i = 0; i < count; i++) counts[i] = p->kref[(size_t)i];
     return chz::OK;
 }
 

@@ -373,12 +373,130 @@ __device__ __forceinline__ void zdivide(const PassArgs &p, const SolveArgs &sa, 
     }
 }
 
+// Exchange a double with the other lane of the pair (lanes 2a <-> 2a + 1):
+// a DPP quad permutation [1, 0, 3, 2] of both 32-bit halves.
+__device__ __forceinline__ double swap_pair(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)bits, 0xB1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0xB1, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The fused z pass of a periodic z axis (R2C forward, divide, C2R inverse)
+// for N = 512 in FP64 on tiles of 16 z-lines x 32 threads, radix plan
+// 8 x 8 x 4.  The generic tile body moves every point through LDS between
+// all phases; here the first stage of the forward FFT and the last stage of
+// the inverse FFT stay in registers, which takes a quarter of the tile's LDS
+// traffic (the staging write and its read back, the last stage's write and
+// the store's read) and two barriers off the pass.  Arithmetic, operation
+// order and results are those of the generic body.
+//
+// Thread t = l + 2 c + 32 h (l = t & 1) loads slots 2h + l + 32 i (i = 0..15)
+// of line c: the lane pair (2h, 2h + 1) holds the real and imaginary parts
+// of points h + 16 i, i.e. stage-1 butterflies h (points h + 32 r: i even) and
+// h + 16 (i odd); one DPP swap of 8 values gives the even lane butterfly h
+// and the odd lane butterfly h + 16, complex.  The last stage (radix 4,
+// Ns = 64) runs butterflies h + 16 l and h + 16 l + 32 per lane, whose
+// outputs are points h + 16 (l + 2 m); a second swap leaves each lane the
+// slot parity it stores (the real parts on even slots, the imaginary parts,
+// negated by the C2R unfold, on odd slots), at slots 2h + l + 32 i again.
+template <bool FULL>
+__device__ __forceinline__ void zmid_periodic_512(const PassArgs &p, const SolveArgs &sa, int a_tile, int b, int nlines,
+                                                  unsigned char *smem, int t, bool tables)
+{
+    typedef double2 V;
+    constexpr int N = 512, M = 256, L = 16, TPL = 32;
+    V *buf = reinterpret_cast<V *>(smem);
+    V *twl = buf + (M + 1) * L;
+    double *red = reinterpret_cast<double *>(twl + M);
+    double *ezl = red + TPL * L;
+    V *wkl = reinterpret_cast<V *>(ezl + (N + 2));
+    const V *tw = reinterpret_cast<const V *>(p.tw);
+    if (tables) {
+        stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
+        const double *ezg = sa.eig + (sa.NX + 2) + (sa.NY + 2);
+        for (int i = t; i < N + 2; i += TPL * L) ezl[i] = ezg[i];
+        for (int i = t; i <= M / 2; i += TPL * L) wkl[i] = tw[4 * i];
+    }
+    const int l = t & 1, cp = (t >> 1) & (L - 1), h = t >> 5;
+    const bool cok = FULL || cp < nlines;
+    // the R2C input window is slots 0..N-1: 16 chunks of 32 slots per line
+    const double *src = reinterpret_cast<const double *>(p.src) + (long long)a_tile * p.s_sa + (long long)b * p.s_sb;
+    const unsigned lane = (unsigned)(cp * (int)p.s_sa) + (unsigned)(2 * h + l) * (unsigned)p.s_sp;
+    const long long step = (long long)TPL * p.s_sp;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = cok ? (src + i * step)[lane] : 0.0;
+    // forward stage 1 (radix 8, Ns = 1) on butterfly h + 16 l: outputs at 8 b + q
+    {
+        V x[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const double got = swap_pair(l ? v[2 * r] : v[2 * r + 1]);
+            x[r] = l ? mk<V>(got, v[2 * r + 1]) : mk<V>(v[2 * r], got);
+        }
+        Dft<8>::run(x, 0, 1);
+        const int bf = h + 16 * l;
+#pragma unroll
+        for (int q = 0; q < 8; q++) buf[(8 * bf + q) * L + cp] = x[q];
+    }
+    __syncthreads();
+    // stages 2 and 3, the split / divide / combine and inverse stages 1 and 2
+    // on the tile in LDS (line t % L, sub-thread t / L)
+    const STile<double, L, false> tl{buf, t % L};
+    const int u = t / L;
+    fft_stage<8, 8>(tl, u, TPL, M, 8, twl);
+    fft_stage<4, 8>(tl, u, TPL, M, 64, twl);
+    periodic_split_divide<8, double, STile<double, L, false>, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
+    fft_stage<8, 8>(tl, u, TPL, M, 1, twl);
+    fft_stage<8, 8>(tl, u, TPL, M, 8, twl);
+    // inverse stage 3 (radix 4, Ns = 64, twiddle stride 1) into registers
+    V y[8];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const int bb = h + 16 * l + 32 * e;   // j = bb (bb < Ns)
+        V z[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) z[r] = buf[(bb + 64 * r) * L + cp];
+#pragma unroll
+        for (int r = 1; r < 4; r++) z[r] = cmul(z[r], twl[r * bb]);
+        Dft<4>::run(z, 0, 1);
+#pragma unroll
+        for (int q = 0; q < 4; q++) y[e + 2 * q] = z[q];   // point bb + 64 q = h + 16 l + 32 (e + 2 q)
+    }
+    double o[16];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const double got = swap_pair(l ? y[m].x : y[m].y);
+        o[2 * m] = l ? got : y[m].x;
+        o[2 * m + 1] = l ? y[m].y : got;
+    }
+    if (cok) {
+        const double sc = l ? -p.scale : p.scale;   // C2R unfold: odd slots negated
+        double *dst = reinterpret_cast<double *>(p.dst) + (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
+        const unsigned dl = (unsigned)(cp * (int)p.d_sa) + (unsigned)(2 * h + l) * (unsigned)p.d_sp;
+        const long long dstep = (long long)TPL * p.d_sp;
+#pragma unroll
+        for (int i = 0; i < 16; i++) (dst + i * dstep)[dl] = o[i] * sc;
+    }
+}
+
 // One pass of the solve or the transform on a tile (op as BatchOp: rows,
 // columns, solve pass 1, fused z pass).
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
                                     int b, int nlines, unsigned char *smem, int t, bool tables = true)
 {
+    if constexpr (OP == B_ZMID && std::is_same<T, double>::value && N == 512 && TPL == 32 && L == 16 && EPT == 8 &&
+                  PLAN == 0x040808ull) {
+        // periodic z with the whole line in the windows: the register-staged form
+        if (p.kind == K_R2C && sa.kind2 == K_C2R && !sa.stencil19 && p.in_lo == 0 && p.in_hi == N - 1 &&
+            p.st_lo == 0 && p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
+            zmid_periodic_512<FULL>(p, sa, a_tile, b, nlines, smem, t, tables);
+            return;
+        }
+    }
     // row tiles are swizzled where the chunk step allows it (32k threads per line, 8 or 16 lines)
     constexpr bool SW = OP != B_COLS && OP != B_ZMID && TPL % 32 == 0 && (L == 8 || L == 16);
     typedef Io<T, N, TPL, L, SW> IO;

@@ -12,6 +12,7 @@
 //   4  y columns:   inverse y                    (in place)
 //   5  rows of ex:  inverse x, x ratio          -> f (unknown box)
 // plus face kernels for the periodic duplicate nodes: 80 bytes per cell.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -94,6 +95,43 @@ std::vector<double> eig_tables(const SolveGeom &g)
         }
     }
     return t;
+}
+
+// Spectral condition number max|e + lambda| / min|e + lambda| of the 7-point
+// operator, e running over the sums e_x + e_y + e_z of the axis eigenvalues
+// of every mode (the modes of each code: the same angles as eig_tables).
+// The minimum comes from a nearest-value search of the sorted z eigenvalues
+// for every (x, y) pair; infinite for a singular operator.
+double spectral_condition(const SolveGeom &g)
+{
+    const double pi = 3.14159265358979323846;
+    std::vector<double> e[3];
+    for (int a = 0; a < 3; a++) {
+        const int N = g.N[a];
+        int m0 = 0, m1 = N;
+        double scale = pi / N, shift = 0;
+        switch (g.bc[a]) {
+            case 0: m1 = N / 2; scale = 2 * pi / N; break;
+            case 1: m0 = 1; m1 = N - 1; break;
+            case 2: m0 = 1; shift = -0.5 * pi / N; break;
+            case 4: m1 = N - 1; shift = 0.5 * pi / N; break;
+            default: break;
+        }
+        for (int m = m0; m <= m1; m++) e[a].push_back((2 * std::cos(m * scale + shift) - 2) * g.ih2[a]);
+        std::sort(e[a].begin(), e[a].end());
+    }
+    double lo = INFINITY;
+    for (double x : e[0])
+        for (double y : e[1]) {
+            const double want = -(x + y + g.lambda);
+            auto it = std::lower_bound(e[2].begin(), e[2].end(), want);
+            if (it != e[2].end()) lo = std::min(lo, std::fabs(x + y + *it + g.lambda));
+            if (it != e[2].begin()) lo = std::min(lo, std::fabs(x + y + *(it - 1) + g.lambda));
+        }
+    const double s0 = e[0].front() + e[1].front() + e[2].front() + g.lambda;
+    const double s1 = e[0].back() + e[1].back() + e[2].back() + g.lambda;
+    const double hi = std::max(std::fabs(s0), std::fabs(s1));
+    return lo > 0 ? hi / lo : INFINITY;
 }
 
 const double *solver_eig(Context &ctx, const SolveGeom &g)

@@ -24,6 +24,8 @@ int make_geom(SolveGeom &g, const double dom[6], const int N[3], const int bc[3]
               const double *const faces[6]);
 // slot eigenvalue tables x | y | z of a geometry (host FP64) and their cached device copy
 std::vector<double> eig_tables(const SolveGeom &g);
+// max|e + lambda| / min|e + lambda| over the modes of the 7-point operator (inf if singular)
+double spectral_condition(const SolveGeom &g);
 const double *solver_eig(Context &ctx, const SolveGeom &g);
 void fill_solve_args(SolveArgs &sa, const SolveGeom &g, const double *f, const double *eig);
 

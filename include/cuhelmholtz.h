@@ -163,7 +163,11 @@ typedef struct chz_problem {
 /* Plan a batched solve of `count` independent problems (any mix of sizes and
  * boundary codes): precision 64 = plain FP64 (refine must be 0), precision
  * 32 = FP32 transforms + `refine` FP64 refinement steps as in
- * chz_helmholtz_solve_mixed (singular problems then need refine == 0).  The
+ * chz_helmholtz_solve_mixed (singular problems then need refine == 0).
+ * refine < 0 (precision 32): each problem gets its own count, at most
+ * -refine: 1 where the spectral condition number max|e + lambda| /
+ * min|e + lambda| of its operator is at most 1e5, 2 up to 1e7, 3 beyond
+ * (chz_batch_refine_counts reports them).  The
  * plan owns one workspace per problem, device tables of per-problem pass
  * descriptors and its own copies of the twiddle and eigenvalue tables it
  * reads, and holds the whole solve captured as one hipGraph: one
@@ -198,6 +202,10 @@ int chz_batch_solve_cached(const chz_problem *problems, int count, int precision
 
 /* Launch count of one batched solve and the plan's device memory. */
 int chz_batch_info(void *plan, int *launches, long long *work_bytes, long long *table_bytes);
+
+/* Refinement steps of each of the plan's `count` problems (all equal unless
+ * the plan was made with refine < 0).  No reference counterpart. */
+int chz_batch_refine_counts(void *plan, int *counts, int count);
 
 /* Free the plan (waits for the plan's last solve or rebind to finish). */
 int chz_batch_destroy(void *plan);

@@ -244,3 +244,54 @@ def test_512_codes_340_vs_hw3crt(lam):
     e = relnp(got, want)
     print(f"512^3 lambda {lam}: {e:.2e} against hw3crt")
     assert e <= TOL, e
+
+
+def axis_eigs(bc, N, h):
+    """(2 cos(angle) - 2) / h^2 over the modes of one axis: code 0 m = 0..N/2
+    at 2 pi m / N; 1 m = 1..N-1, 3 m = 0..N at pi m / N; 2 m = 1..N at
+    pi m / N - pi / 2N; 4 m = 0..N-1 at pi m / N + pi / 2N."""
+    m = {0: np.arange(N // 2 + 1), 1: np.arange(1, N), 2: np.arange(1, N + 1), 3: np.arange(N + 1),
+         4: np.arange(N)}[bc]
+    ang = {0: 2 * np.pi * m / N, 1: np.pi * m / N, 2: np.pi * m / N - np.pi / (2 * N), 3: np.pi * m / N,
+           4: np.pi * m / N + np.pi / (2 * N)}[bc]
+    return (2 * np.cos(ang) - 2) / h ** 2
+
+
+def kappa(d):
+    """max|e + lambda| / min|e + lambda| over the modes of one problem (numpy)."""
+    ex, ey, ez = (axis_eigs(d["bc"][a], d["n"][a], 1.0 / d["n"][a]) for a in range(3))
+    s = np.abs((ex[:, None, None] + ey[None, :, None] + ez[None, None, :]).ravel() + d["lam"])
+    return s.max() / s.min()
+
+
+def test_batched_per_problem_refinement():
+    """FP32 batch with per-problem refinement counts (refine < 0): the counts
+    follow the spectral-condition rule (1 up to 1e5, 2 up to 1e7, 3 beyond),
+    and every problem is within 1e-10 of its FP64 batched solve.  The problems:
+    the stratified subset plus the workload's worst-conditioned ones."""
+    descs = W.batched_problems(count=2048, seed=2)
+    kap = np.array([kappa(d) for d in descs])
+    pick = sorted(set(stratified(descs)) | set(np.argsort(-kap)[:8].tolist()))
+    sub = [descs[i] for i in pick]
+    f64, f32, probs64, probs32 = [], [], [], []
+    for d in sub:
+        f, bcs = W.problem_arrays(d)
+        bt = [None if b is None else torch.from_numpy(b).to(dev()) for b in bcs]
+        for lst, pl in ((f64, probs64), (f32, probs32)):
+            t = torch.from_numpy(f).to(dev())
+            lst.append(t)
+            pl.append((t, d["n"], d["bc"], d["lam"], bt))
+    b64 = C.BatchSolver(probs64, 64)
+    b32 = C.BatchSolver(probs32, 32, refine=-3)
+    b64.solve()
+    b32.solve()
+    torch.cuda.synchronize()
+    counts = b32.refine_counts()
+    for k, i in enumerate(pick):
+        want = 1 if kap[i] <= 1e5 else 2 if kap[i] <= 1e7 else 3
+        assert counts[k] == want, (descs[i], kap[i], counts[k])
+        e = reltorch(f32[k], f64[k])
+        assert e <= 1e-10, (descs[i], kap[i], counts[k], e)
+    print(f"{len(sub)} problems, refinement counts {sorted(set(counts))}")
+    b64.close()
+    b32.close()
