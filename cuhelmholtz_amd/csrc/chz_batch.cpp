@@ -74,7 +74,9 @@ struct BatchPlan {
     char *staging = nullptr;              // pinned copy of the rewritable part (rebind uploads)
     std::vector<GroupRec> groups;
     size_t c_off[3] = {0, 0, 0};          // periodic copy lists in the image
-    std::vector<size_t> ws_off, u_off;    // workspace / iterate offsets in work
+    std::vector<size_t> ws_off, u_off;    // workspace / FP64 iterate offsets in work
+    std::vector<size_t> u32_off;          // FP32 first iterate offsets (mixed solves on lean-I/O kernels)
+    std::vector<char> u32;                // problem keeps its first iterate in FP32
     // the plan's own twiddle and eigenvalue tables (image offsets): the
     // descriptors point here, never at the library's caches
     std::map<std::pair<int, int>, size_t> tw_off;   // (N, precision)
@@ -183,12 +185,19 @@ static int build_specs(Context &ctx, const chz_problem *probs, const BatchPlan &
             double *ex = reinterpret_cast<double *>(wbase + plan.ws_off[i]);
             if (!(rc = solve_passes<double>(ctx, g, f, ex, nullptr, nullptr, v))) rc = final_pass<double>(ctx, g, ex, nullptr, f, v);
         } else {
+            // as helmholtz_mixed: the first iterate in FP32 where the plan keeps it so
             float *ex = reinterpret_cast<float *>(wbase + plan.ws_off[i]);
-            double *U = refine > 0 ? reinterpret_cast<double *>(wbase + plan.u_off[i]) : nullptr;
+            const bool u32 = plan.u32[i] != 0;
+            double *U = refine > 0 && (!u32 || refine > 1) ? reinterpret_cast<double *>(wbase + plan.u_off[i]) : nullptr;
+            double *U0 = u32 ? reinterpret_cast<double *>(wbase + plan.u32_off[i]) : nullptr;   // FP32 data
             if (!(rc = solve_passes<float>(ctx, g, f, ex, nullptr, nullptr, v)))
-                rc = final_pass<float>(ctx, g, ex, nullptr, refine > 0 ? U : f, v);
-            for (int it = 1; it <= refine && rc == OK; it++)
-                if (!(rc = solve_passes<float>(ctx, g, f, ex, U, nullptr, v))) rc = final_pass<float>(ctx, g, ex, U, it == refine ? f : U, v);
+                rc = final_pass<float>(ctx, g, ex, nullptr, refine == 0 ? f : u32 ? U0 : U, v, nullptr, u32 ? 2 : 0);
+            for (int it = 1; it <= refine && rc == OK; it++) {
+                const bool from32 = u32 && it == 1;
+                const double *Uin = from32 ? U0 : U;
+                if (!(rc = solve_passes<float>(ctx, g, f, ex, Uin, nullptr, v, nullptr, from32 ? 1 : 0)))
+                    rc = final_pass<float>(ctx, g, ex, Uin, it == refine ? f : U, v, nullptr, from32 ? 1 : 0);
+            }
         }
         if (rc) return fail(rc, "problem " + std::to_string(i) + ": " + last_error());
     }
@@ -247,7 +256,8 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
 
     std::vector<SolveGeom> geo(count);
     const size_t esz = prec == 64 ? 8 : 4;
-    std::vector<size_t> ws_off(count), u_off(count, 0);
+    std::vector<size_t> ws_off(count), u_off(count, 0), u32_off(count, 0);
+    std::vector<char> u32(count, 0);
     std::vector<int> kref(count, refine);
     size_t total = 0;
     for (int i = 0; i < count; i++) {
@@ -262,9 +272,15 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
                                              "are solved in FP64 or alone in mixed precision");
         ws_off[i] = total;
         total += align256(esz * (size_t)ws_row(q.N[0], esz) * (q.N[1] + 2) * (q.N[2] + 2));
-        if (prec == 32 && kref[i] > 0) {
+        const size_t nodes = (size_t)(q.N[0] + 1) * (q.N[1] + 1) * (q.N[2] + 1);
+        if (prec == 32 && kref[i] > 0 && iterate_f32_ok(ctx, geo[i], true)) {
+            u32[i] = 1;
+            u32_off[i] = total;
+            total += align256(sizeof(float) * nodes);
+        }
+        if (prec == 32 && kref[i] > 0 && (!u32[i] || kref[i] > 1)) {
             u_off[i] = total;
-            total += align256(sizeof(double) * (size_t)(q.N[0] + 1) * (q.N[1] + 1) * (q.N[2] + 1));
+            total += align256(sizeof(double) * nodes);
         }
     }
     BatchPlan *plan = new BatchPlan();
@@ -276,6 +292,8 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     plan->work_bytes = total;
     plan->ws_off = ws_off;
     plan->u_off = u_off;
+    plan->u32_off = u32_off;
+    plan->u32 = u32;
     for (int i = 0; i < count; i++) plan->keys.push_back(key_of(geo[i], probs[i]));
     if (hipMalloc(&plan->work, total) != hipSuccess) {
         delete plan;
@@ -449,6 +467,11 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
             delete plan;
             return fail(ERR_UNSUPPORTED, "no batched kernel for this pass");
         }
+        for (size_t k = 0; !ln && k < gr.probs.size(); k++)
+            if (specs[gr.probs[k]][gr.step].d.p.u_f32) {
+                delete plan;
+                return fail(ERR_INVALID, "internal: FP32 iterate on a pass without a lean-I/O kernel");
+            }
         l.threads = q.d.p.L * q.d.p.TPL;
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l.fn, l.threads, l.lds) != hipSuccess || occ <= 0) occ = 1;

@@ -32,27 +32,34 @@ bool line_windows_ok(const PassArgs &p, int N)
     return in(p.in_lo, p.in_hi) && in(p.st_lo, p.st_hi) && in(p.out_lo, p.out_hi);
 }
 
+const FixedEntry *line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan)
+{
+    // CHZ_NO_LINE / CHZ_NO_FIXED in the environment: none
+    static const bool off = getenv("CHZ_NO_FIXED") != nullptr || getenv("CHZ_NO_LINE") != nullptr;
+    if (off) return nullptr;
+    typedef const FixedEntry *(*Tab)(int *);
+    static const Tab t64[] = {line_table_f64, line_small_table_f64, line_mid_table_f64};
+    static const Tab t32[] = {line_table_f32, line_small_table_f32, line_mid_table_f32};
+    for (int q = 0; q < 3; q++) {
+        int n = 0;
+        const FixedEntry *t = (prec == 64 ? t64 : t32)[q](&n);
+        for (int i = 0; i < n; i++) {
+            const FixedEntry &e = t[i];
+            if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
+        }
+    }
+    return nullptr;
+}
+
 const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
                                const PassArgs *p)
 {
     static const bool off = getenv("CHZ_NO_FIXED") != nullptr;
     if (off) return nullptr;
-    // the lean-I/O kernels first (CHZ_NO_LINE in the environment: the plain
-    // fixed-shape kernels)
-    static const bool no_line = getenv("CHZ_NO_LINE") != nullptr;
+    // the lean-I/O kernels first where the pass windows allow them
+    if (p && line_windows_ok(*p, N))
+        if (const FixedEntry *e = line_kernel(prec, op, N, ept, tpl, lines, plan)) return e;
     int n = 0;
-    if (!no_line && p && line_windows_ok(*p, N)) {
-        typedef const FixedEntry *(*Tab)(int *);
-        static const Tab t64[] = {line_table_f64, line_small_table_f64, line_mid_table_f64};
-        static const Tab t32[] = {line_table_f32, line_small_table_f32, line_mid_table_f32};
-        for (int q = 0; q < 3; q++) {
-            const FixedEntry *t = (prec == 64 ? t64 : t32)[q](&n);
-            for (int i = 0; i < n; i++) {
-                const FixedEntry &e = t[i];
-                if (e.op == op && e.N == N && e.ept == ept && e.tpl == tpl && e.lines == lines && e.plan == plan) return &e;
-            }
-        }
-    }
     const FixedEntry *t = prec == 64 ? fixed_table_f64(&n) : fixed_table_f32(&n);
     for (int i = 0; i < n; i++) {
         const FixedEntry &e = t[i];

@@ -39,6 +39,8 @@ struct PassArgs {
     int L, TPL;
     int xcds;                     // XCD count for the tile swizzle (1 = dispatch order)
     int bfast;                    // 1: consecutive tiles step through b first (neighbour planes in time)
+    int u_f32;                    // residual / update passes of the mixed solve (lean-I/O kernels only):
+                                  // bit 0 the iterate read is FP32, bit 1 the iterate written is FP32
     unsigned long long plan;
     const void *tw;
     double scale;
@@ -133,6 +135,9 @@ struct FixedEntry {
 // with lean tile I/O are preferred where the pass windows allow them
 const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
                                const PassArgs *p = nullptr);
+// the lean-I/O single-problem kernel of a pass shape, nullptr if none is compiled
+// (or CHZ_NO_LINE / CHZ_NO_FIXED is set)
+const FixedEntry *line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
 // fixed-shape batched kernel (arguments as k_batch), nullptr if none
 const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
 // batched kernel with lean tile I/O (arguments as k_batch), nullptr if none

@@ -160,7 +160,9 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     // pass), rounded to T.  Interior chunks load f, U and its four y / z
     // neighbour rows at per-lane row pointers plus constant offsets, and the x
     // neighbours at +-1 element; the x-boundary rules only enter the edge chunks.
-    __device__ __forceinline__ static void stage_xres(const SolveArgs &sa, const double *U, V *buf, int nlines,
+    // U: the iterate, FP64 or (the first iterate of a mixed solve) FP32
+    template <typename UT>
+    __device__ __forceinline__ static void stage_xres(const SolveArgs &sa, const UT *U, V *buf, int nlines,
                                                       int a_tile, int k, int t)
     {
         // rows of 32+ lanes (a power of two): the x neighbours of the interior
@@ -173,8 +175,10 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         const long long fx = sa.NX + 1, fy = sa.NY + 1, fxy = fx * fy;
         const long long c0 = (long long)a_tile * fx + (long long)k * fxy;   // wave-uniform
         const unsigned rowoff = (unsigned)((rok ? r : 0) * (int)fx);
-        const double *fr = sa.f + c0 + rowoff, *ur = U + c0 + rowoff;        // row starts
-        const double *fl = fr + l, *ul = ur + l;                              // this lane's slot l
+        const double *fr = sa.f + c0 + rowoff;                               // row starts
+        const UT *ur = U + c0 + rowoff;
+        const double *fl = fr + l;                                            // this lane's slot l
+        const UT *ul = ur + l;
         // y / z neighbour rows of U: offsets and 0 / 1 weights (periodic wrap, Neumann mirror, Dirichlet 0)
         long long dm[2], dp[2];
         double wm[2], wp[2];
@@ -191,7 +195,7 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
             else if (bc == 0) { dp[d] = -(long long)(hi - lo) * st[d]; wp[d] = 1.0; }
             else if ((bc == 2 || bc == 3) && hi > lo) { dp[d] = -st[d]; wp[d] = 1.0; }
         }
-        const double *uym = ul + dm[0], *uyp = ul + dp[0], *uzm = ul + dm[1], *uzp = ul + dp[1];
+        const UT *uym = ul + dm[0], *uyp = ul + dp[0], *uzm = ul + dm[1], *uzp = ul + dp[1];
         const RowFolds rf(sa, j, k);
         const int xs = sa.s[0], xe = sa.e[0], bx = sa.bc[0];
         const int xm_end = bx == 0 ? xe : ((bx == 3 || bx == 4) && xe > xs ? xs + 1 : xs);
@@ -289,18 +293,18 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
     // ---- store ---------------------------------------------------------------
     // last pass of a mixed-precision solve (body_xupd of the generic pass):
     // out = uin + scale * delta in FP64 over slots [st_lo, st_hi] of every row
-    // (uin == nullptr: out = scale * delta)
-    template <bool FULL>
-    __device__ __forceinline__ static void store_rows_upd(const PassArgs &p, V *buf, int nlines, const double *uin,
-                                                          double *out, long long obase, int t, bool negodd)
+    // (uin == nullptr: out = scale * delta), the iterates FP64 or FP32
+    template <bool FULL, typename UIT, typename UOT>
+    __device__ __forceinline__ static void store_rows_upd(const PassArgs &p, V *buf, int nlines, const UIT *uin,
+                                                          UOT *out, long long obase, int t, bool negodd)
     {
         const int r = t / TPL, l = t % TPL;
         if (!FULL && r >= nlines) return;
         const double sc = (negodd && (l & 1)) ? -p.scale : p.scale;
         const int hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
         const unsigned lane = (unsigned)(r * (int)p.d_sa + l);
-        double *orow = out + obase + lane;
-        const double *irow = uin ? uin + obase + lane : nullptr;
+        UOT *orow = out + obase + lane;
+        const UIT *irow = uin ? uin + obase + lane : nullptr;
         const T *lp = lds_base(buf, r, l);
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
@@ -308,8 +312,8 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
             if (s >= NS) continue;
             if (!interior(i) && !(s >= p.st_lo && s <= hi)) continue;
             double v = (double)lp[i * lds_step] * sc;
-            if (irow) v += irow[i * TPL];
-            orow[i * TPL] = v;
+            if (irow) v += (double)irow[i * TPL];
+            orow[i * TPL] = (UOT)v;
         }
     }
     // slots [st_lo, st_hi] of every line, value * scale inside [out_lo, out_hi], 0 outside
@@ -525,7 +529,10 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     if constexpr (OP == B_ROWS || OP == B_XUPD) IO::template stage_rows<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_COLS || OP == B_ZMID) IO::template stage_cols<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_XFWD) IO::stage_xfwd(sa, buf, nlines, a_tile, b, t);
-    if constexpr (OP == B_XRES) IO::stage_xres(sa, u_in, buf, nlines, a_tile, b, t);
+    if constexpr (OP == B_XRES) {
+        if (p.u_f32 & 1) IO::stage_xres(sa, reinterpret_cast<const float *>(u_in), buf, nlines, a_tile, b, t);
+        else IO::stage_xres(sa, u_in, buf, nlines, a_tile, b, t);
+    }
     __syncthreads();
     PassArgs q = p;
     // the last transform of the pass; a C2R one leaves its unfold (a sign
@@ -562,7 +569,16 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     }
     const long long dbase = (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
     T *dst = reinterpret_cast<T *>(p.dst) + dbase;
-    if constexpr (OP == B_XUPD) IO::template store_rows_upd<FULL>(q, buf, nlines, u_in, u_out, dbase, t, last == K_C2R);
+    if constexpr (OP == B_XUPD) {
+        const bool neg = last == K_C2R;
+        if (p.u_f32 & 2)   // the first iterate of a mixed solve, stored in FP32
+            IO::template store_rows_upd<FULL>(q, buf, nlines, reinterpret_cast<const float *>(u_in),
+                                              reinterpret_cast<float *>(u_out), dbase, t, neg);
+        else if (p.u_f32 & 1)
+            IO::template store_rows_upd<FULL>(q, buf, nlines, reinterpret_cast<const float *>(u_in), u_out, dbase, t, neg);
+        else
+            IO::template store_rows_upd<FULL>(q, buf, nlines, u_in, u_out, dbase, t, neg);
+    }
     else if constexpr (OP == B_ROWS || OP == B_XFWD || OP == B_XRES)
         IO::template store_rows<FULL>(q, buf, nlines, dst, t, last == K_C2R);
     else IO::template store_cols<FULL>(q, buf, nlines, dst, t, last == K_C2R);

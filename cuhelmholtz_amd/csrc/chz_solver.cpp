@@ -186,7 +186,7 @@ static int check_spec(const PassSpec &q, long long svol, long long dvol, const c
 // refinement step of the mixed-precision solve).
 template <typename T>
 int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
-                 std::vector<PassSpec> &out, const SolveTables *tabs)
+                 std::vector<PassSpec> &out, const SolveTables *tabs, int u_f32)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
@@ -229,6 +229,7 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
         q.nb = g.e[2] - g.s[2] + 1;
         q.d.sa = sa;
         q.d.u_in = U;
+        p.u_f32 = U ? (u_f32 & 1) : 0;
         // the residual reads the z-neighbour rows of U: step through z first so
         // they are the rows of the tiles running at the same time (L2 hits)
         p.bfast = U ? 1 : 0;
@@ -298,7 +299,7 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
 // ratio * delta in FP64 (Uin == nullptr: out = ratio * delta).
 template <typename T>
 int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v,
-               const SolveTables *tabs)
+               const SolveTables *tabs, int u_f32)
 {
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
@@ -324,6 +325,7 @@ int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin,
     p.st_lo = g.s[0]; p.st_hi = g.e[0];
     p.out_lo = g.s[0]; p.out_hi = g.e[0];
     p.scale = g.ratio;
+    p.u_f32 = u_f32;
     q.nb = g.e[2] - g.s[2] + 1;
     q.d.u_in = Uin;
     q.d.u_out = out;
@@ -375,13 +377,24 @@ static int run_specs(Context &ctx, const std::vector<PassSpec> &v, hipStream_t s
 }
 
 template int solve_passes<double>(Context &, const SolveGeom &, double *, double *, const double *, const double *,
-                                  std::vector<PassSpec> &, const SolveTables *);
+                                  std::vector<PassSpec> &, const SolveTables *, int);
 template int solve_passes<float>(Context &, const SolveGeom &, double *, float *, const double *, const double *,
-                                 std::vector<PassSpec> &, const SolveTables *);
+                                 std::vector<PassSpec> &, const SolveTables *, int);
 template int final_pass<double>(Context &, const SolveGeom &, const double *, const double *, double *, std::vector<PassSpec> &,
-                                const SolveTables *);
+                                const SolveTables *, int);
 template int final_pass<float>(Context &, const SolveGeom &, const float *, const double *, double *, std::vector<PassSpec> &,
-                               const SolveTables *);
+                               const SolveTables *, int);
+
+bool iterate_f32_ok(Context &ctx, const SolveGeom &g, bool batched)
+{
+    AxisCfg cr, cu;
+    const int N = g.N[0];
+    if (!ctx.config(N, 32, ROLE_XFWD, &cr) || !ctx.config(N, 32, ROLE_ROWS, &cu)) return false;
+    if (batched)
+        return batch_line_kernel(32, B_XRES, N, cr.EPT, cr.TPL, cr.L, cr.plan) &&
+               batch_line_kernel(32, B_XUPD, N, cu.EPT, cu.TPL, cu.L, cu.plan);
+    return line_kernel(32, B_XRES, N, cr.EPT, cr.TPL, cr.L, cr.plan) && line_kernel(32, B_XUPD, N, cu.EPT, cu.TPL, cu.L, cu.plan);
+}
 
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st)
 {
@@ -438,10 +451,15 @@ int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const 
     std::lock_guard<std::recursive_mutex> lock(ctx.mutex());
     WsLease ws(ctx, st);
     const size_t wbytes = sizeof(float) * (size_t)ws_row(N[0], sizeof(float)) * (N[1] + 2) * (N[2] + 2);
-    const size_t ubytes = sizeof(double) * (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
+    const size_t nodes = (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
+    // the first iterate (the FP32 solve, ratio x FP32 values) is kept in FP32
+    // where the lean-I/O residual and update kernels run: 4 bytes per node
+    // less to write and to read back; the later iterates are FP64
+    const bool u32 = refine > 0 && iterate_f32_ok(ctx, g, false);
     float *ex = reinterpret_cast<float *>(ws.get(wbytes, 1));
-    double *U = refine > 0 ? reinterpret_cast<double *>(ws.get(ubytes, 2)) : nullptr;
-    if (!ex || (refine > 0 && !U)) return fail(ERR_ALLOC, "workspace allocation failed");
+    float *U0 = u32 ? reinterpret_cast<float *>(ws.get(sizeof(float) * nodes, 7)) : nullptr;
+    double *U = refine > 0 && (!u32 || refine > 1) ? reinterpret_cast<double *>(ws.get(sizeof(double) * nodes, 2)) : nullptr;
+    if (!ex || (u32 && !U0) || (refine > 0 && (!u32 || refine > 1) && !U)) return fail(ERR_ALLOC, "workspace allocation failed");
     PassTimer timer(ctx, st);
     // singular case: PERTRB (weighted sum of the folded rhs) in FP64, once
     double *acc = nullptr;
@@ -455,13 +473,17 @@ int helmholtz_mixed(const double dom[6], const int N[3], const int bc[3], const 
         if (e != hipSuccess) return fail(ERR_DEVICE, std::string("PERTRB launch: ") + hipGetErrorString(e));
     }
     std::vector<PassSpec> v;
+    double *first = refine == 0 ? f : u32 ? reinterpret_cast<double *>(U0) : U;
     if ((rc = solve_passes<float>(ctx, g, f, ex, nullptr, nullptr, v)) ||
-        (rc = final_pass<float>(ctx, g, ex, nullptr, refine > 0 ? U : f, v)))
+        (rc = final_pass<float>(ctx, g, ex, nullptr, first, v, nullptr, u32 ? 2 : 0)))
         return rc;
-    for (int it = 1; it <= refine; it++)
-        if ((rc = solve_passes<float>(ctx, g, f, ex, U, acc ? acc + 1 : nullptr, v)) ||
-            (rc = final_pass<float>(ctx, g, ex, U, it == refine ? f : U, v)))
+    for (int it = 1; it <= refine; it++) {
+        const bool from32 = u32 && it == 1;
+        const double *Uin = from32 ? reinterpret_cast<const double *>(U0) : U;
+        if ((rc = solve_passes<float>(ctx, g, f, ex, Uin, acc ? acc + 1 : nullptr, v, nullptr, from32 ? 1 : 0)) ||
+            (rc = final_pass<float>(ctx, g, ex, Uin, it == refine ? f : U, v, nullptr, from32 ? 1 : 0)))
             return rc;
+    }
     if ((rc = run_specs(ctx, v, st))) return rc;
     if (acc) {
         // the corrections never change the 0-mode of the FP32 solve (rounding

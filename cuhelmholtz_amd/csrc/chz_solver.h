@@ -48,13 +48,19 @@ struct SolveTables {
 };
 
 // passes 1-4 of a solve into workspace ex (rhs read from f; U: residual of U instead)
+// (u_f32: the iterate U is FP32, see PassArgs::u_f32)
 template <typename T>
 int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const double *U, const double *pertrb,
-                 std::vector<PassSpec> &out, const SolveTables *tabs = nullptr);
+                 std::vector<PassSpec> &out, const SolveTables *tabs = nullptr, int u_f32 = 0);
 // inverse x pass into out (FP64: the solution; FP32: out = Uin + delta)
+// (u_f32: PassArgs::u_f32 bits, Uin / out FP32 where set)
 template <typename T>
 int final_pass(Context &ctx, const SolveGeom &g, const T *ex, const double *Uin, double *out, std::vector<PassSpec> &v,
-               const SolveTables *tabs = nullptr);
+               const SolveTables *tabs = nullptr, int u_f32 = 0);
+// the FP32 residual and update passes of a mixed solve of g run on the
+// lean-I/O kernels (single-problem or batched), the only ones that read and
+// write FP32 iterates
+bool iterate_f32_ok(Context &ctx, const SolveGeom &g, bool batched);
 int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st);
 int periodic_copies(const SolveGeom &g, double *f, hipStream_t st);
 
