@@ -47,6 +47,8 @@ def lib():
         L.chz_set_timing.argtypes = [i]
         L.chz_set_axis_config.argtypes = [i] * 6
         L.chz_load_tuning.argtypes = [ctypes.c_char_p]
+        L.chz_radix_plan.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.chz_compiled_kernels.argtypes = [ctypes.POINTER(i), i]
         L.chz_get_timing.argtypes = [ctypes.POINTER(dp), i, ctypes.c_char_p, i]
         L.chz_axis_config.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                       ctypes.POINTER(ctypes.c_ulonglong)]
@@ -530,7 +532,47 @@ def set_axis_config(N, prec, role, ept, tpl, lines):
 
 
 def reset_axis_configs():
+    """Forget every pinned or loaded configuration (back to the built-in choice)."""
     lib().chz_reset_axis_configs()
+
+
+def tuning_table_path(arch="gfx950"):
+    """The tuning table the library loads at start-up (beside the library)."""
+    return os.path.join(HERE, "tuning", f"{arch}.txt")
+
+
+def load_tuning(path=None):
+    """Pin every configuration of a tuning table (default: the shipped one);
+    returns how many rows were valid."""
+    n = lib().chz_load_tuning((path or tuning_table_path()).encode())
+    _check(min(n, 0))
+    return n
+
+
+def radix_plan(M, ept, tpl):
+    """The library's radix plan, one byte per stage radix (first stage in the
+    low byte; a 1 is a direct stage of the remaining factor), for an M-point
+    FFT with at most ept points per thread and tpl threads per line, or None
+    if the shape cannot run (host only)."""
+    p = ctypes.c_ulonglong()
+    if lib().chz_radix_plan(int(M), int(ept), int(tpl), ctypes.byref(p)) != 0:
+        return None
+    return p.value
+
+
+def compiled_kernels():
+    """The compiled fixed-shape pass kernels as tuples (form, prec, op, N, ept,
+    tpl, lines, plan), form 0 single-problem lean tile I/O, 1 its batched form,
+    2 single-problem fixed shape, 3 batched fixed shape (host only)."""
+    L = lib()
+    n = L.chz_compiled_kernels(None, 0)
+    buf = (ctypes.c_int * (9 * n))()
+    L.chz_compiled_kernels(buf, n)
+    out = []
+    for k in range(n):
+        r = buf[9 * k:9 * k + 9]
+        out.append(tuple(r[:7]) + (((r[8] & 0xffffffff) << 32) | (r[7] & 0xffffffff),))
+    return out
 
 
 def in_window(kind, N):

@@ -1,8 +1,10 @@
 """Build libcuhelmholtz_amd.so in-tree with hipcc for gfx950.
 
-Each source under csrc/ is compiled to an object in csrc/../_build (in
-parallel), then linked into cuhelmholtz_amd/libcuhelmholtz_amd.so; the
-command-line drivers (drivers/*.cpp) are linked against it into bin/.
+The fixed-shape kernel tables are first generated from the tuning tables
+(kernelgen.generate into _build/gen); each source under csrc/ and each
+generated source is compiled to an object in _build (in parallel), then
+linked into cuhelmholtz_amd/libcuhelmholtz_amd.so; the command-line drivers
+(drivers/*.cpp) are linked against it into bin/.
 Usage: python -m cuhelmholtz_amd.build  (or build() from Python).
 """
 import concurrent.futures as cf
@@ -20,7 +22,9 @@ DRIVERS = os.path.join(HERE, "drivers")
 BIN = os.path.join(HERE, "bin")
 ARCH = os.environ.get("CHZ_OFFLOAD_ARCH", "gfx950")
 
-COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
+GEN = os.path.join(BUILD, "gen")
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include"),
+          "-I", CSRC]
 # packed FP32 instructions are disabled for all device code; multiply-adds are
 # fused within a source expression only (-ffp-contract=on), so a tile body
 # rounds the same way in every kernel it is inlined into (a batched problem
@@ -37,11 +41,14 @@ def hipcc():
 
 
 def _deps(path, seen=None):
-    """Local headers a source includes, transitively (#include "...")."""
+    """Local headers a source includes, transitively (#include "...", beside
+    the source or in csrc/)."""
     seen = set() if seen is None else seen
     with open(path) as f:
         for m in re.finditer(r'#include "([^"]+)"', f.read()):
             h = os.path.normpath(os.path.join(os.path.dirname(path), m.group(1)))
+            if not os.path.exists(h):
+                h = os.path.join(CSRC, m.group(1))
             if h not in seen and os.path.exists(h):
                 seen.add(h)
                 _deps(h, seen)
@@ -65,7 +72,10 @@ def _compile(src, incremental=False):
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    """csrc/ sources plus the kernel tables generated from the tuning tables."""
+    from cuhelmholtz_amd import kernelgen
+    gen = kernelgen.generate(GEN, arch=ARCH)
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp"))) + gen
 
 
 def build(verbose=True, incremental=False):

@@ -123,13 +123,22 @@ const void *periodic_copy_kernel();
 const void *batch_kernel_f64(int op, int ept);
 const void *batch_kernel_f32(int op, int ept);
 const void *batch_periodic_kernel();
+// forms of a compiled fixed-shape kernel: single-problem k_line / batched
+// k_line_batch (lean tile I/O, even threads per line), single k_fixed /
+// batched k_batch_fixed (odd threads per line)
+enum FixedForm { F_LINE = 0, F_LINE_BATCH = 1, F_FIXED = 2, F_FIXED_BATCH = 3 };
 // a compiled fixed-shape pass kernel: (precision, op, N, elements/thread, threads/line, lines, plan)
 struct FixedEntry {
     int prec, op, N, ept, tpl, lines;
     unsigned long long plan;
     const void *fn;
     unsigned lds_extra;   // LDS bytes beyond axis_pass_lds (tables the kernel stages)
+    int form;             // FixedForm
 };
+// the compiled shapes: tables generated from the tuning tables at build time
+// (cuhelmholtz_amd/kernelgen.py writes them and their index)
+int gen_table_count();
+const FixedEntry *gen_table(int q, int *n);
 // fixed-shape kernel of a pass (arguments PassArgs, SolveArgs, const double *, double *), nullptr if
 // none (CHZ_NO_FIXED in the environment disables them); given the pass arguments, the kernels
 // with lean tile I/O are preferred where the pass windows allow them
@@ -146,6 +155,10 @@ const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
 const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
 // the windows of a pass contain slots 2..N-2 (precondition of the lean-I/O kernels)
 bool line_windows_ok(const PassArgs &p, int N);
+// rows of 9 ints per compiled fixed-shape kernel (form, prec, op, N, ept, tpl,
+// lines, plan low 32 bits, plan high 32 bits) into rows[0 .. 9 max_rows);
+// returns the kernel count
+int compiled_kernels(int *rows, int max_rows);
 hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
                         int prec, int nb, hipStream_t s);
 

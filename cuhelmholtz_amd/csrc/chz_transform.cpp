@@ -239,6 +239,16 @@ int chz_load_tuning(const char *path)
     return n;
 }
 
+int chz_radix_plan(int M, int ept, int tpl, unsigned long long *plan)
+{
+    unsigned long long p = 0;
+    if (!chz::radix_plan(M, ept, tpl, &p)) return chz::ERR_INVALID;
+    if (plan) *plan = p;
+    return chz::OK;
+}
+
+int chz_compiled_kernels(int *rows, int max_rows) { return chz::compiled_kernels(rows, max_rows); }
+
 int chz_version(void) { return 100; }
 
 void chz_set_timing(int on) { chz::Context::get().set_timing(on != 0); }

@@ -241,6 +241,22 @@ void chz_reset_axis_configs(void);
  * $CHZ_TUNE_FILE or tuning/<gfx arch>.txt next to the shared library. */
 int chz_load_tuning(const char *path);
 
+/* The radix plan the library derives for an M-point FFT with at most ept
+ * points per thread and tpl threads per line (one byte per stage, first stage
+ * in the low byte); -1 if that shape cannot run.  Host only (no device call).
+ * No reference counterpart (cuFFT plans are opaque). */
+int chz_radix_plan(int M, int ept, int tpl, unsigned long long *plan);
+
+/* The compiled fixed-shape pass kernels (generated at build time from the
+ * tuning tables tuning/<arch>.txt and tuning/<arch>.alt.txt): for kernel i,
+ * rows[9 i .. 9 i + 8] = form (0 single-problem lean tile I/O, 1 its batched
+ * form, 2 single-problem fixed shape, 3 batched fixed shape), precision, pass
+ * op (0 rows, 1 columns, 2 solve pass 1, 3 fused z pass, 4 residual, 5
+ * update), N, points per thread, threads per line, lines per tile, radix plan
+ * low and high 32 bits.  Writes at most max_rows rows (rows may be NULL) and
+ * returns the kernel count.  Host only.  No reference counterpart. */
+int chz_compiled_kernels(int *rows, int max_rows);
+
 int chz_version(void);
 
 /* Per-pass timing, the counterpart of the reference's TIME_TEST splits
