@@ -18,6 +18,14 @@
 #include "chz_axis_kernels.h"
 
 namespace chz {
+
+// LDS bytes a kernel stages beyond the tile, FFT twiddles and reduction
+// scratch: the fused z pass keeps the z eigenvalues and w^0..w^(M/2)
+template <typename T, int N, int OP> constexpr unsigned line_lds_extra()
+{
+    return OP == B_ZMID ? (unsigned)((N + 2) * sizeof(double) + (N / 4 + 1) * 2 * sizeof(T)) : 0u;
+}
+
 namespace line {
 
 // Line permutation of point row k for the row passes, chosen so that the 32
@@ -486,6 +494,171 @@ __device__ __forceinline__ void zmid_periodic_512(const PassArgs &p, const Solve
     }
 }
 
+// A wave-local fence for LDS data exchanged between the lanes of one wave:
+// the wave executes its LDS instructions in order, so only the compiler has
+// to be kept from moving the accesses across it.
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Lane (within a 16-lane line group) that holds the FFT outputs k = k1 + 16 k2
+// of residue k1 after the first transpose: residues k1 and 16 - k1 (the
+// pairs (k, M - k) of the real-FFT split) sit on lanes 2p and 2p + 1, and the
+// self-paired residues 0 and 8 on lanes 0 and 1.
+__device__ __forceinline__ int wz_lane_of(int k1) { return k1 == 0 ? 0 : k1 == 8 ? 1 : k1 < 8 ? 2 * k1 : 2 * (16 - k1) + 1; }
+__device__ __forceinline__ int wz_res_of(int l) { return l == 0 ? 0 : l == 1 ? 8 : (l & 1) ? 16 - (l >> 1) : l >> 1; }
+
+// The fused z pass of a periodic z axis (R2C forward, divide, C2R inverse)
+// for N = 512 in FP64 with wave-resident transforms: a tile of 16 x-columns
+// on 256 threads.  The tile is staged through LDS with whole 128-byte rows per
+// slot (16 columns x 4 slots per load instruction), column-major (point k of
+// column c at tb[257 c + k]); then each wave owns 4 columns, each z line
+// (M = 256 complex points) held in 16 lanes x 16 registers, and both 256-point
+// FFTs are four-step 16 x 16:
+//   Z_{k1 + 16 k2} = sum_q W16^{q k2} W256^{q k1} sum_m z_{q + 16 m} W16^{m k1}
+// (lane q: a 16-point DFT over its registers m, the twiddle, a transpose
+// across the 16 lanes of the line through the line's own column of the tile,
+// a second 16-point DFT).  The split / divide / combine pairs k with M - k,
+// which the transpose puts on the lane pair (2p, 2p + 1) (wz_lane_of), so one
+// DPP swap brings each lane its partner values.  The inverse FFT is the same
+// four-step on the conj-combined values.  Two workgroup barriers in all
+// (after staging, before the store); every other exchange stays in one wave.
+// Arithmetic of the split / divide / combine as periodic_split_divide (for
+// k > M/2 the same formulas with w^k = e^{2 pi i k / N} from the table).
+template <bool FULL, bool S19>
+__device__ __forceinline__ void zmid_periodic_512_wave(const PassArgs &p, const SolveArgs &sa, int a_tile, int b, int nlines,
+                                                       unsigned char *smem, int t, bool tables)
+{
+    typedef double2 V;
+    constexpr int N = 512, M = 256, CS = M + 1;
+    // LDS: tile (16 columns x 257 points) | W256^j | w^k, k < M | z eigenvalues
+    V *tb = reinterpret_cast<V *>(smem);
+    V *twl = tb + 16 * CS;
+    V *wkl = twl + M;
+    double *ezl = reinterpret_cast<double *>(wkl + M);
+    const V *tw = reinterpret_cast<const V *>(p.tw);
+    if (tables) {
+        stage_twiddles(twl, tw + TwOffsets::fftw(N), M, 256, t);
+        for (int i = t; i < M; i += 256) wkl[i] = tw[4 * i];   // half[4k] = e^{2 pi i k / N}
+        const double *ezg = sa.eig + (sa.NX + 2) + (sa.NY + 2);
+        for (int i = t; i < N + 2; i += 256) ezl[i] = ezg[i];
+    }
+    // staging: thread (column sc, row sr) moves slots sr + 16 i, i = 0..31 (the R2C input window 0..N-1)
+    const int sc = t & 15, sr = t >> 4;
+    double *tr = reinterpret_cast<double *>(tb) + 2 * CS * sc;   // real slots of column sc
+    {
+        const bool ok = FULL || sc < nlines;
+        const double *src = reinterpret_cast<const double *>(p.src) + (long long)a_tile * p.s_sa + (long long)b * p.s_sb;
+        const unsigned off = (unsigned)(sc * (int)p.s_sa) + (unsigned)sr * (unsigned)p.s_sp;
+        const long long st = 16LL * p.s_sp;
+        double v[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) v[i] = ok ? (src + i * st)[off] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 32; i++) tr[sr + 16 * i] = v[i];
+    }
+    __syncthreads();
+    const int lane = t & 63, wv = t >> 6;
+    const int cw = lane >> 4, l = lane & 15;   // line of the wave, lane within the line
+    const int c = 4 * wv + cw;                 // column of the tile
+    const bool cok = FULL || c < nlines;
+    V *blk = tb + CS * c;                      // this line's column: also its transpose block
+    V x[16];
+    // lane l: points l + 16 m
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = blk[l + 16 * m];
+    wave_lds_fence();
+    // forward step 1: DFT over m, twiddle W256^{l k1}, transpose to lane wz_lane_of(k1), register l
+    Dft<16>::run(x, 0, 1);
+#pragma unroll
+    for (int k1 = 1; k1 < 16; k1++) x[k1] = cmul(x[k1], twl[l * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+        const int d = wz_lane_of(k1);
+        blk[d * 16 + (l ^ d)] = x[k1];
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = blk[l * 16 + (j ^ l)];
+    wave_lds_fence();
+    // forward step 2: DFT over q -> Z_{k1 + 16 k2}, k1 = wz_res_of(l), in x[k2]
+    Dft<16>::run(x, 0, 1);
+    const int k1 = wz_res_of(l);
+    // partner values Z_{M - k}: lanes 2..15 from the pair lane (register
+    // 15 - k2 of the swapped copy), lane 1 (k1 = 8) from its own register
+    // 15 - k2, lane 0 (k1 = 0) from its own register (16 - k2) mod 16
+    // (selects on the scalar components: a select of the vector structs would
+    // select their addresses and put the arrays in scratch memory)
+    V zm[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const double sx = swap_pair(x[r].x), sy = swap_pair(x[r].y);
+        zm[r].x = l >= 2 ? sx : x[r].x;
+        zm[r].y = l >= 2 ? sy : x[r].y;
+    }
+    // split, divide, combine (periodic_split_divide's formulas for mode k = k1 + 16 k2)
+    {
+        const int xc = a_tile + c;
+        const double *ex = sa.eig, *ey = sa.eig + (sa.NX + 2);
+        const LineDivide<S19> dv(sa, cok ? ex[xc] : 0.0, cok ? ey[b] : 0.0);
+        const bool m0xy = sa.singular && cok && (sa.bc[0] == 0 ? xc < 2 : xc == 0) && (sa.bc[1] == 0 ? b < 2 : b == 0);
+        const double h = 0.5;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; k2++) {
+            const int k = k1 + 16 * k2;
+            V Zm;
+            Zm.x = l == 0 ? zm[(16 - k2) & 15].x : zm[15 - k2].x;
+            Zm.y = l == 0 ? zm[(16 - k2) & 15].y : zm[15 - k2].y;
+            const V Zk = x[k2], w = wkl[k];
+            const V E = mk<V>(h * (Zk.x + Zm.x), h * (Zk.y - Zm.y));
+            const V O = mk<V>(h * (Zk.y + Zm.y), -h * (Zk.x - Zm.x));
+            const V P = mk<V>(w.x * O.x + w.y * O.y, w.x * O.y - w.y * O.x);
+            V Xk = mk<V>(E.x + P.x, E.y + P.y), Xm = mk<V>(E.x - P.x, -(E.y - P.y));
+            const double rk = (m0xy && k == 0) ? 0.0 : dv(ezl[2 * k]);
+            const double rm = dv(ezl[2 * (M - k)]);
+            Xk = mk<V>(Xk.x * rk, Xk.y * rk);
+            Xm = mk<V>(Xm.x * rm, Xm.y * rm);
+            if (k == 0) {
+                Xk.y = 0;
+                Xm.y = 0;
+            }
+            const V E2 = mk<V>(Xk.x + Xm.x, Xk.y - Xm.y);
+            const V D2 = mk<V>(Xk.x - Xm.x, Xk.y + Xm.y);
+            const V O2 = cmul(D2, w);
+            x[k2] = mk<V>(E2.x - O2.y, -(E2.y + O2.x));
+        }
+    }
+    // inverse step 1: DFT over k2, twiddle W256^{k1 n2}, transpose to lane n2, register k1
+    Dft<16>::run(x, 0, 1);
+#pragma unroll
+    for (int n2 = 1; n2 < 16; n2++) x[n2] = cmul(x[n2], twl[k1 * n2]);
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) blk[n2 * 16 + (k1 ^ n2)] = x[n2];
+    wave_lds_fence();
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = blk[l * 16 + (j ^ l)];
+    wave_lds_fence();
+    // inverse step 2: DFT over k1 -> Y_{n2 + 16 n1} (n2 = l) in x[n1], back into the column
+    Dft<16>::run(x, 0, 1);
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) blk[l + 16 * n1] = x[n1];
+    __syncthreads();
+    // store: C2R unfold (x[2 n] = Re Y_n, x[2 n + 1] = -Im Y_n) times the ratio
+    if (FULL || sc < nlines) {
+        double *dst = reinterpret_cast<double *>(p.dst) + (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
+        const unsigned off = (unsigned)(sc * (int)p.d_sa) + (unsigned)sr * (unsigned)p.d_sp;
+        const long long st = 16LL * p.d_sp;
+        const double scl = (sr & 1) ? -p.scale : p.scale;   // slot parity = sr parity
+#pragma unroll
+        for (int i = 0; i < 32; i++) (dst + i * st)[off] = tr[sr + 16 * i] * scl;
+    }
+}
+
+// LDS bytes of zmid_periodic_512_wave
+constexpr unsigned zmid_wave_lds() { return (unsigned)((16 * 257 + 256 + 256) * 16 + 514 * 8); }
+
 // One pass of the solve or the transform on a tile (op as BatchOp: rows,
 // columns, solve pass 1, fused z pass).
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
@@ -498,6 +671,19 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         if (p.kind == K_R2C && sa.kind2 == K_C2R && !sa.stencil19 && p.in_lo == 0 && p.in_hi == N - 1 &&
             p.st_lo == 0 && p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
             zmid_periodic_512<FULL>(p, sa, a_tile, b, nlines, smem, t, tables);
+            return;
+        }
+    }
+    if constexpr (OP == B_ZMID && std::is_same<T, double>::value && N == 512 && TPL == 16 && L == 16 && EPT == 16 &&
+                  PLAN == 0x1010ull) {
+        // the tile's LDS (FP64 tile, twiddles, reduction scratch, z tables) holds the wave-resident form's
+        static_assert(zmid_wave_lds() <= (unsigned)(((N / 2 + 1) * L + N / 2) * 16 + TPL * L * 8) +
+                                             line_lds_extra<double, N, B_ZMID>(),
+                      "LDS of the wave-resident fused z pass");
+        if (p.kind == K_R2C && sa.kind2 == K_C2R && p.in_lo == 0 && p.in_hi == N - 1 && p.st_lo == 0 &&
+            p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
+            if (sa.stencil19) zmid_periodic_512_wave<FULL, true>(p, sa, a_tile, b, nlines, smem, t, tables);
+            else zmid_periodic_512_wave<FULL, false>(p, sa, a_tile, b, nlines, smem, t, tables);
             return;
         }
     }
@@ -588,12 +774,6 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
 
 // Waves per SIMD that the LDS of one tile leaves room for (160 KB per CU, at
 // most 4): the register budget of the kernel follows from it.
-// LDS bytes a kernel stages beyond the tile, FFT twiddles and reduction
-// scratch: the fused z pass keeps the z eigenvalues and w^0..w^(M/2)
-template <typename T, int N, int OP> constexpr unsigned line_lds_extra()
-{
-    return OP == B_ZMID ? (unsigned)((N + 2) * sizeof(double) + (N / 4 + 1) * 2 * sizeof(T)) : 0u;
-}
 
 template <typename T, int N, int TPL, int L, int OP> constexpr int line_occupancy()
 {

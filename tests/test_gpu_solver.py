@@ -118,6 +118,35 @@ def test_long_axis_every_bc_vs_oracle(axis):
             assert rel(ft.cpu().numpy(), O.helmholtz(f, n, bc, lam, bcs, stencil=19)) < TOL, (n, bc, lam, 19)
 
 
+def test_zmid_512_every_compiled_shape_vs_oracle():
+    """Every compiled launch shape of the 512-cell fused z pass (FP64), the
+    wave-resident periodic form (16 x 16 points per line) among them, on z
+    lines of every code, full and partial 16-column tiles (NX = 18 with a
+    19-column DCT-I window, NX = 32 periodic), the singular periodic /
+    Neumann case and the 19-point operator, against the oracle."""
+    from cuhelmholtz_amd import autotune as A
+    shapes = A.compiled_shapes(512, 64, C.ROLE_ZMID)
+    assert (16, 16, 16) in shapes
+    rng = np.random.default_rng(77)
+    cases = [((18, 6, 512), (3, 1, 0), -2.3), ((32, 8, 512), (0, 0, 0), 0.0), ((32, 8, 512), (0, 4, 0), 0.5),
+             ((18, 6, 512), (2, 3, 4), -1.0), ((10, 6, 512), (4, 2, 1), -7.0), ((32, 6, 512), (1, 0, 3), 0.0)]
+    try:
+        for ept, tpl, lines in shapes:
+            C.set_axis_config(512, 64, C.ROLE_ZMID, ept, tpl, lines)
+            for n, bc, lam in cases:
+                f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+                bcs = faces(rng, n, bc)
+                for stencil in (7, 19):
+                    ft = torch.from_numpy(f.copy()).to(dev())
+                    C.helmholtz_solve(ft, n, bc, lam, bcs, stencil=stencil)
+                    torch.cuda.synchronize()
+                    want = O.helmholtz(f, n, bc, lam, bcs, stencil=stencil)
+                    assert rel(ft.cpu().numpy(), want) < TOL, ((ept, tpl, lines), n, bc, lam, stencil)
+    finally:
+        C.reset_axis_configs()
+        C.load_tuning()
+
+
 @pytest.mark.parametrize("bc", [(0, 0, 0), (3, 3, 3), (0, 3, 0), (3, 0, 3)])
 def test_singular_zero_mode(bc):
     rng = np.random.default_rng(2)
