@@ -230,7 +230,7 @@ int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines,
                     unsigned long long *plan);
 
 /* Pin a launch configuration (used by the autotuner); -1 if it cannot run
- * (no radix plan for ept/tpl, more than 256 threads, or LDS over the limit). */
+ * (no radix plan for ept/tpl, more than 512 threads per tile, or LDS over the limit). */
 int chz_set_axis_config(int N, int prec, int role, int ept, int tpl, int lines);
 
 /* Forget pinned / cached configurations (back to the built-in choice). */
