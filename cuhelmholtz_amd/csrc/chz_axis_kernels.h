@@ -602,10 +602,11 @@ __global__ void __launch_bounds__(256) k_periodic_copy(T *f, int NX, int NY, int
 // ---------------------------------------------------------------------------
 // Batched passes.  A batched solve runs, per pass of the solve, one launch per
 // size class (problems sharing the cell count along the pass axis, hence the
-// launch configuration).  Each launch is persistent: its workgroups walk the
-// class's list of work items (problem, tile).  The items are cut into 8
-// contiguous chunks, one per XCD (workgroup b runs on XCD b % 8), so the
-// tiles of one problem share an L2.  The pass arguments of every problem come
+// launch configuration).  The workgroups of a launch walk the class's list of
+// work items (problem, tile), cut into 8 contiguous chunks, one per XCD
+// (workgroup b runs on XCD b % 8), so the tiles of one problem share an L2;
+// the host launches one workgroup per item (measured faster than a grid of
+// resident workgroups walking several).  The pass arguments of every problem come
 // from a descriptor table.
 // ---------------------------------------------------------------------------
 // Shape overrides of a batched pass: none (generic), or compile-time N,

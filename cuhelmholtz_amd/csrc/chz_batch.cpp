@@ -4,9 +4,9 @@
 // A plan is built once from the problem list.  Every problem's solve is the
 // same pass sequence as the single solve (5 passes for FP64; 5 per FP32
 // solve / refinement for the mixed form), so step s of all problems is
-// launched together: one persistent launch per size class (problems sharing
-// the cell count -- hence the launch configuration -- along the pass axis).
-// Each launch walks a work list of (problem, tile) items and reads every
+// launched together: one launch per size class (problems sharing the cell
+// count -- hence the launch configuration -- along the pass axis), one
+// workgroup per (problem, tile) item of its work list, reading every
 // problem's pass arguments from a device descriptor table.  The periodic
 // copies follow as three launches (x, y, z faces, in that order, as in the
 // single solve).  The whole sequence is captured into one hipGraph at plan
@@ -55,7 +55,7 @@ struct ProbKey {
 
 typedef std::tuple<int, int, int, int, unsigned long long> LaunchKey;   // N, ept, TPL, L, plan
 
-// step s of the problems sharing one launch configuration: one persistent launch
+// step s of the problems sharing one launch configuration: one launch
 struct GroupRec {
     int step;
     LaunchKey key;
@@ -313,8 +313,6 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
     for (int i = 0; i < count; i++) nsteps = std::max(nsteps, specs[i].size());
 
     // group step s of all problems by launch configuration
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->dev) != hipSuccess || cus <= 0) cus = 256;
     std::vector<GroupRec> &groups = plan->groups;
     for (size_t s = 0; s < nsteps; s++) {
         std::map<LaunchKey, size_t> idx;
@@ -473,10 +471,12 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
                 return fail(ERR_INVALID, "internal: FP32 iterate on a pass without a lean-I/O kernel");
             }
         l.threads = q.d.p.L * q.d.p.TPL;
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l.fn, l.threads, l.lds) != hipSuccess || occ <= 0) occ = 1;
-        long long grid = std::min<long long>(nitems[gi], (long long)occ * cus);
-        if (grid >= 8) grid -= grid % 8;   // whole XCD rounds (item chunks per XCD)
+        // one workgroup per (problem, tile) item, in whole XCD rounds (item chunks
+        // per XCD): workgroups that start and end independently overlap one
+        // tile's memory phase with another's compute better than a persistent
+        // grid of resident workgroups walking the items (2048-problem workload:
+        // FP64 22.6 -> 20.9 ms, FP32 + refinement 40.1 -> 34.2 ms)
+        long long grid = (nitems[gi] + 7) / 8 * 8;
         l.grid = (int)std::max<long long>(grid, 1);
         l.items = reinterpret_cast<const BatchItem *>(tbase + i_off[gi]);
         l.nitems = nitems[gi];

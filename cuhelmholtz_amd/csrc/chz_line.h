@@ -806,10 +806,11 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) 
     else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
 }
 
-// Batched pass with lean tile I/O: the persistent work-item loop of the
-// batched passes (k_batch_fixed: items cut into 8 contiguous chunks, one per
-// XCD) around the tile body of k_line.  Same precondition on the windows of
-// every problem of the launch.
+// Batched pass with lean tile I/O: the work-item loop of the batched passes
+// (k_batch_fixed: items cut into 8 contiguous chunks, one per XCD; launched
+// with one workgroup per item, a smaller grid walks several) around the tile
+// body of k_line.  Same precondition on the windows of every problem of the
+// launch.
 template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
 __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
     k_line_batch(const BatchItem *__restrict__ items, int nitems, const BatchDesc *__restrict__ descs)
