@@ -562,16 +562,18 @@ def radix_plan(M, ept, tpl):
 
 def compiled_kernels():
     """The compiled fixed-shape pass kernels as tuples (form, prec, op, N, ept,
-    tpl, lines, plan), form 0 single-problem lean tile I/O, 1 its batched form,
-    2 single-problem fixed shape, 3 batched fixed shape (host only)."""
+    tpl, lines, plan, ks), form 0 single-problem lean tile I/O, 1 its batched
+    form, 2 single-problem fixed shape, 3 batched fixed shape; ks the kind
+    selector (-1 every kind, else the pass kind or the fused z pass's boundary
+    code) (host only)."""
     L = lib()
     n = L.chz_compiled_kernels(None, 0)
-    buf = (ctypes.c_int * (9 * n))()
+    buf = (ctypes.c_int * (10 * n))()
     L.chz_compiled_kernels(buf, n)
     out = []
     for k in range(n):
-        r = buf[9 * k:9 * k + 9]
-        out.append(tuple(r[:7]) + (((r[8] & 0xffffffff) << 32) | (r[7] & 0xffffffff),))
+        r = buf[10 * k:10 * k + 10]
+        out.append(tuple(r[:7]) + (((r[8] & 0xffffffff) << 32) | (r[7] & 0xffffffff), r[9]))
     return out
 
 

@@ -19,7 +19,7 @@ the right answer independently of any launch configuration, plus agreement
 with the built-in configuration's result:
   * solves: the normalised 7-point residual |f - A u| / (|A| |u| + |f|) of
     the discrete system (cuhelmholtz_amd/checks.py) below 1e-13 (FP64) or
-    1e-6 (FP32 without refinement);
+    2e-5 (FP32 without refinement: about 3e-6 at 512^3);
   * transforms: the round trip DCT-III(DCT-II(x)) = (N/2)^3 x per axis
     triple, to 1e-12 (FP64) or 1e-5 (FP32);
   * batched solves: the residual of every problem of the batch.
@@ -42,7 +42,7 @@ from cuhelmholtz_amd import kernelgen as G
 HERE = os.path.dirname(os.path.abspath(__file__))
 EPTS = G.EPTS
 LINES = (1, 2, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32, 48, 64, 96, 128, 256, 512)
-RES_TOL = {64: 1e-13, 32: 1e-6}
+RES_TOL = {64: 1e-13, 32: 2e-5}
 AGREE_TOL = {64: 1e-12, 32: 1e-5}
 
 
@@ -68,8 +68,8 @@ def compiled_shapes(N, prec, role):
     """(ept, tpl, lines) of N with a compiled single-problem kernel for every
     pass of the role."""
     have = {}
-    for form, p, op, n, ept, tpl, lines, plan in C.compiled_kernels():
-        if p == prec and n == N and form in (G.F_LINE, G.F_FIXED):
+    for form, p, op, n, ept, tpl, lines, plan, ks in C.compiled_kernels():
+        if p == prec and n == N and form in (G.F_LINE, G.F_FIXED) and ks < 0:
             have.setdefault((ept, tpl, lines), set()).add(op)
     need = set(G.role_ops(role, prec))
     return sorted(k for k, ops in have.items() if need <= ops)

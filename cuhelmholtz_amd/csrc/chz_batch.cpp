@@ -451,7 +451,13 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         BatchLaunch l;
         l.lds = axis_pass_lds(q.prec, q.d.p.M, q.d.p.L, q.d.p.TPL);
         // lean tile I/O where the windows of every problem of the group allow it
-        const FixedEntry *ln = batch_line_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+        // a kind-specialised kernel where every problem of the group has the same kind
+        int ks = pass_kind_sel(q.op, q.d.p, &q.d.sa);
+        for (size_t k = 1; ks >= 0 && k < gr.probs.size(); k++) {
+            const PassSpec &qk = specs[gr.probs[k]][gr.step];
+            if (pass_kind_sel(qk.op, qk.d.p, &qk.d.sa) != ks) ks = -1;
+        }
+        const FixedEntry *ln = batch_line_kernel(q.prec, q.op, q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan, ks);
         for (size_t k = 0; ln && k < gr.probs.size(); k++)
             if (!line_windows_ok(specs[gr.probs[k]][gr.step].d.p, q.N)) ln = nullptr;
         if (ln) {

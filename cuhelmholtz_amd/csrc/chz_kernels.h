@@ -134,7 +134,12 @@ struct FixedEntry {
     const void *fn;
     unsigned lds_extra;   // LDS bytes beyond axis_pass_lds (tables the kernel stages)
     int form;             // FixedForm
+    int ks;               // -1: any kind (read from the pass); else the kind of the pass (the z code of a fused z pass)
 };
+// kind selector of a pass for the kind-specialised kernels: the pass kind, or
+// for the fused z pass the boundary code of its (forward, inverse) kind pair
+// (-1 if the pair is not one of a boundary code's)
+int pass_kind_sel(int op, const PassArgs &p, const SolveArgs *sa);
 // the compiled shapes: tables generated from the tuning tables at build time
 // (cuhelmholtz_amd/kernelgen.py writes them and their index)
 int gen_table_count();
@@ -143,21 +148,22 @@ const FixedEntry *gen_table(int q, int *n);
 // none (CHZ_NO_FIXED in the environment disables them); given the pass arguments, the kernels
 // with lean tile I/O are preferred where the pass windows allow them
 const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
-                               const PassArgs *p = nullptr);
+                               const PassArgs *p = nullptr, int ks = -1);
 // the lean-I/O single-problem kernel of a pass shape, nullptr if none is compiled
 // (or CHZ_NO_LINE / CHZ_NO_FIXED is set)
-const FixedEntry *line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+const FixedEntry *line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan, int ks = -1);
 // fixed-shape batched kernel (arguments as k_batch), nullptr if none
 const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
 // batched kernel with lean tile I/O (arguments as k_batch), nullptr if none
 // (CHZ_NO_LINE / CHZ_NO_FIXED in the environment: none); usable when every
 // problem of the launch passes line_windows_ok
-const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
+const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,
+                                    int ks = -1);
 // the windows of a pass contain slots 2..N-2 (precondition of the lean-I/O kernels)
 bool line_windows_ok(const PassArgs &p, int N);
-// rows of 9 ints per compiled fixed-shape kernel (form, prec, op, N, ept, tpl,
-// lines, plan low 32 bits, plan high 32 bits) into rows[0 .. 9 max_rows);
-// returns the kernel count
+// rows of 10 ints per compiled fixed-shape kernel (form, prec, op, N, ept, tpl,
+// lines, plan low 32 bits, plan high 32 bits, kind selector) into
+// rows[0 .. 10 max_rows); returns the kernel count
 int compiled_kernels(int *rows, int max_rows);
 hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs *sa, const void *u_in, void *u_out,
                         int prec, int nb, hipStream_t s);

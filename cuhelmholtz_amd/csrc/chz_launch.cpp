@@ -29,7 +29,7 @@ hipError_t launch_fixed(const FixedEntry *fx, const PassArgs &p, const SolveArgs
 
 hipError_t launch_axis_pass(const PassArgs &p, int prec, int ept, bool rows, int nb, hipStream_t s)
 {
-    if (const FixedEntry *fx = fixed_kernel(prec, rows ? B_ROWS : B_COLS, p.N, ept, p.TPL, p.L, p.plan, &p))
+    if (const FixedEntry *fx = fixed_kernel(prec, rows ? B_ROWS : B_COLS, p.N, ept, p.TPL, p.L, p.plan, &p, p.kind))
         return launch_fixed(fx, p, nullptr, nullptr, nullptr, prec, nb, s);
     const void *fn = prec == 64 ? (rows ? axis_kernel_f64_rows(ept) : axis_kernel_f64_cols(ept))
                                 : (rows ? axis_kernel_f32_rows(ept) : axis_kernel_f32_cols(ept));

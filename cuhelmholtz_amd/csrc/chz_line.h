@@ -659,16 +659,27 @@ __device__ __forceinline__ void zmid_periodic_512_wave(const PassArgs &p, const 
 // LDS bytes of zmid_periodic_512_wave
 constexpr unsigned zmid_wave_lds() { return (unsigned)((16 * 257 + 256 + 256) * 16 + 514 * 8); }
 
+// Forward / inverse kind of the fused z pass for boundary code c (0..4).
+__host__ __device__ constexpr int zmid_fwd_kind(int c) { return c == 0 ? K_R2C : c == 1 ? K_DST_1 : c == 2 ? K_DST_3 : c == 3 ? K_DCT_1 : K_DCT_3; }
+__host__ __device__ constexpr int zmid_inv_kind(int c) { return c == 0 ? K_C2R : c == 1 ? K_DST_1 : c == 2 ? K_DST_2 : c == 3 ? K_DCT_1 : K_DCT_2; }
+
 // One pass of the solve or the transform on a tile (op as BatchOp: rows,
-// columns, solve pass 1, fused z pass).
-template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL>
+// columns, solve pass 1, fused z pass).  KS < 0: the kind comes from the pass
+// arguments (one kernel holds every kind's fold / unfold); KS >= 0: the kind
+// is a compile-time constant (the pass kind, or for the fused z pass the z
+// boundary code), so the kernel holds only that kind's code (a kernel with
+// every kind inlined is several hundred KB, and where its hot path lands in
+// the instruction cache changed its speed by half).
+template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL, int KS = -1>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
                                     int b, int nlines, unsigned char *smem, int t, bool tables = true)
 {
+    const int kind = KS < 0 ? p.kind : OP == B_ZMID ? zmid_fwd_kind(KS) : KS;
+    const int kind2 = KS < 0 ? sa.kind2 : OP == B_ZMID ? zmid_inv_kind(KS) : sa.kind2;
     if constexpr (OP == B_ZMID && std::is_same<T, double>::value && N == 512 && TPL == 32 && L == 16 && EPT == 8 &&
                   PLAN == 0x040808ull) {
         // periodic z with the whole line in the windows: the register-staged form
-        if (p.kind == K_R2C && sa.kind2 == K_C2R && !sa.stencil19 && p.in_lo == 0 && p.in_hi == N - 1 &&
+        if (kind == K_R2C && kind2 == K_C2R && !sa.stencil19 && p.in_lo == 0 && p.in_hi == N - 1 &&
             p.st_lo == 0 && p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
             zmid_periodic_512<FULL>(p, sa, a_tile, b, nlines, smem, t, tables);
             return;
@@ -680,7 +691,7 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         static_assert(zmid_wave_lds() <= (unsigned)(((N / 2 + 1) * L + N / 2) * 16 + TPL * L * 8) +
                                              line_lds_extra<double, N, B_ZMID>(),
                       "LDS of the wave-resident fused z pass");
-        if (p.kind == K_R2C && sa.kind2 == K_C2R && p.in_lo == 0 && p.in_hi == N - 1 && p.st_lo == 0 &&
+        if (kind == K_R2C && kind2 == K_C2R && p.in_lo == 0 && p.in_hi == N - 1 && p.st_lo == 0 &&
             p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
             if (sa.stencil19) zmid_periodic_512_wave<FULL, true>(p, sa, a_tile, b, nlines, smem, t, tables);
             else zmid_periodic_512_wave<FULL, false>(p, sa, a_tile, b, nlines, smem, t, tables);
@@ -723,16 +734,16 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     PassArgs q = p;
     // the last transform of the pass; a C2R one leaves its unfold (a sign
     // flip of the odd slots) to the store
-    int last = p.kind;
+    int last = kind;
     if constexpr (OP == B_ZMID) {
-        last = sa.kind2;
-        if (p.kind == K_R2C && sa.kind2 == K_C2R) {
+        last = kind2;
+        if (kind == K_R2C && kind2 == K_C2R) {
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
             if (sa.stencil19) periodic_split_divide<EPT, T, TL, true, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             else periodic_split_divide<EPT, T, TL, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
         } else {
-            line_transform<EPT, T>(p.kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
+            line_transform<EPT, T>(kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
             if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
             else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
             __syncthreads();
@@ -787,7 +798,7 @@ template <typename T, int N, int TPL, int L, int OP> constexpr int line_occupanc
 // Fixed-shape pass kernel with lean tile I/O (same arguments as k_fixed).
 // Precondition (checked by the host): every window of the pass contains
 // slots 2..N-2, which holds for every kind and solve pass.
-template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
+template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, int KS = -1>
 __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) k_line(PassArgs p, SolveArgs sa,
                                                                                   const double *u_in, double *u_out)
 {
@@ -802,8 +813,8 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) 
     tile_of(q, a_tile, b);
     const int nlines = min(L, q.a0 + q.na - a_tile);
     const int t = (int)threadIdx.x;
-    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
-    else line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
+    if (nlines == L) line::run<T, OP, N, TPL, L, PLAN, EPT, true, KS>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
+    else line::run<T, OP, N, TPL, L, PLAN, EPT, false, KS>(q, sa, u_in, u_out, a_tile, b, nlines, smem, t);
 }
 
 // Batched pass with lean tile I/O: the work-item loop of the batched passes

@@ -343,7 +343,7 @@ int launch_spec(Context &ctx, const PassSpec &q, hipStream_t st)
     const PassArgs &p = q.d.p;
     const bool f64 = q.prec == 64;
     hipError_t e = hipErrorInvalidValue;
-    const FixedEntry *fx = fixed_kernel(q.prec, q.op, p.N, q.ept, p.TPL, p.L, p.plan, &p);
+    const FixedEntry *fx = fixed_kernel(q.prec, q.op, p.N, q.ept, p.TPL, p.L, p.plan, &p, pass_kind_sel(q.op, p, &q.d.sa));
     // an FP32 iterate is read / written by the lean-I/O kernels only
     if (p.u_f32 && (!fx || fx->form != F_LINE))
         return fail(ERR_INVALID, std::string("internal: FP32 iterate on the ") + q.name + " pass without a lean-I/O kernel");

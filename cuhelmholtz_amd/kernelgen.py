@@ -22,6 +22,16 @@ zmid: the fused z pass; zcols: the transform's z pass):
     (FP32 batches end with the update pass, so their rows pass is not built);
   * an odd thread count: k_fixed, and for N <= 128 k_batch_fixed.
 
+Rows of the chosen table with N in SPECIALIZE_N (512, the large single
+solves and transforms) are also compiled once per kind the pass can run
+(single-problem lean-I/O form): every kind for the rows and columns passes,
+the forward kinds of the five boundary codes for solve pass 1 and the
+residual pass, their inverse kinds for the update pass, and the five codes'
+(forward, inverse) pairs for the fused z pass.  Such a kernel holds only that
+kind's code (one with every kind inlined is several hundred KB, and its speed
+changed by half with where its hot path landed in the instruction cache); the
+library picks it for a pass of that kind, else the kernel for every kind.
+
 Shapes in neither file run on the generic kernels (run-time radix plan).
 build() calls generate() before compiling; a CPU test checks that the
 library's compiled shapes (chz_compiled_kernels) are exactly kernel_set() of
@@ -45,6 +55,21 @@ BATCH_MAX_N = 128
 LDS_BYTES = 163840
 RADICES = (16, 12, 10, 8, 6, 5, 4, 3, 2)
 KERNELS_PER_FILE = 6
+SPECIALIZE_N = (512,)
+K_DCT_1, K_DCT_2, K_DCT_3, K_DST_1, K_DST_2, K_DST_3, K_R2C, K_C2R = range(8)
+FWD_KINDS = (K_R2C, K_DST_1, K_DST_3, K_DCT_1, K_DCT_3)   # boundary codes 0..4
+INV_KINDS = (K_C2R, K_DST_1, K_DST_2, K_DCT_1, K_DCT_2)
+
+
+def kind_sels(op):
+    """Kind selectors a pass op can run with (the fused z pass: boundary codes)."""
+    if op in (B_ROWS, B_COLS):
+        return tuple(range(8))
+    if op in (B_XFWD, B_XRES):
+        return FWD_KINDS
+    if op == B_XUPD:
+        return INV_KINDS
+    return tuple(range(5))
 
 
 def radix_plan(M, ept, tpl):
@@ -140,28 +165,34 @@ def table_paths(arch="gfx950"):
 
 
 def kernel_set(paths=None, arch="gfx950"):
-    """Sorted, de-duplicated shapes to compile: (form, prec, op, N, ept, tpl, lines, plan)."""
+    """Sorted, de-duplicated kernels to compile: (form, prec, op, N, ept, tpl,
+    lines, plan, ks), ks = -1 for every kind, else the kind selector."""
     out = set()
-    for path in paths or table_paths(arch):
+    paths = paths or table_paths(arch)
+    for pi, path in enumerate(paths):
         for row in read_table(path):
             role, N, prec, ept, tpl, lines = row
             plan = check_row(row, f"{os.path.basename(path)}: ")
             single, batched = (F_LINE, F_LINE_BATCH) if tpl % 2 == 0 else (F_FIXED, F_FIXED_BATCH)
             for op in role_ops(role, prec):
-                out.add((single, prec, op, N, ept, tpl, lines, plan))
+                out.add((single, prec, op, N, ept, tpl, lines, plan, -1))
                 # batched solves end their FP32 passes with the update pass, not a rows pass
                 if N <= BATCH_MAX_N and not (prec == 32 and op == B_ROWS):
-                    out.add((batched, prec, op, N, ept, tpl, lines, plan))
+                    out.add((batched, prec, op, N, ept, tpl, lines, plan, -1))
+                # kind-specialised forms of the chosen large shapes (first table only)
+                if pi == 0 and N in SPECIALIZE_N and single == F_LINE:
+                    for ks in kind_sels(op):
+                        out.add((single, prec, op, N, ept, tpl, lines, plan, ks))
     return sorted(out)
 
 
 def _entry(k):
-    form, prec, op, N, ept, tpl, lines, plan = k
+    form, prec, op, N, ept, tpl, lines, plan, ks = k
     T = "double" if prec == 64 else "float"
     opn = OP_NAMES[op]
     pl = f"0x{plan:x}ull"
     if form == F_LINE:
-        fn = f"k_line<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}>"
+        fn = f"k_line<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}" + (f", {ks}>" if ks >= 0 else ">")
     elif form == F_LINE_BATCH:
         fn = f"k_line_batch<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}>"
     elif form == F_FIXED:
@@ -170,7 +201,7 @@ def _entry(k):
         fn = f"k_batch_fixed<{T}, {ept}, {opn}, {N}, {tpl}, {lines}, {pl}>"
     extra = f"line_lds_extra<{T}, {N}, {opn}>()" if form in (F_LINE, F_LINE_BATCH) else "0u"
     return (f"    {{{prec}, {opn}, {N}, {ept}, {tpl}, {lines}, {pl}, (const void *){fn},\n"
-            f"     {extra}, F_{FORM_NAMES[form].upper()}}},")
+            f"     {extra}, F_{FORM_NAMES[form].upper()}, {ks}}},")
 
 
 def sources(paths=None, arch="gfx950"):
@@ -223,4 +254,4 @@ def generate(out_dir, paths=None, arch="gfx950"):
 
 if __name__ == "__main__":
     for k in kernel_set():
-        print(FORM_NAMES[k[0]], k[1:7], hex(k[7]))
+        print(FORM_NAMES[k[0]], k[1:7], hex(k[7]), k[8])

@@ -249,12 +249,14 @@ int chz_radix_plan(int M, int ept, int tpl, unsigned long long *plan);
 
 /* The compiled fixed-shape pass kernels (generated at build time from the
  * tuning tables tuning/<arch>.txt and tuning/<arch>.alt.txt): for kernel i,
- * rows[9 i .. 9 i + 8] = form (0 single-problem lean tile I/O, 1 its batched
+ * rows[10 i .. 10 i + 9] = form (0 single-problem lean tile I/O, 1 its batched
  * form, 2 single-problem fixed shape, 3 batched fixed shape), precision, pass
  * op (0 rows, 1 columns, 2 solve pass 1, 3 fused z pass, 4 residual, 5
  * update), N, points per thread, threads per line, lines per tile, radix plan
- * low and high 32 bits.  Writes at most max_rows rows (rows may be NULL) and
- * returns the kernel count.  Host only.  No reference counterpart. */
+ * low and high 32 bits, kind selector (-1: every kind; else the kind of the
+ * pass, for a fused z pass the boundary code 0-4).  Writes at most max_rows
+ * rows (rows may be NULL) and returns the kernel count.  Host only.  No
+ * reference counterpart. */
 int chz_compiled_kernels(int *rows, int max_rows);
 
 int chz_version(void);
