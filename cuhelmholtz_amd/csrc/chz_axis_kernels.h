@@ -72,12 +72,25 @@ struct RowFolds {
             else { fp[3] = sa.bce + j * fx; fc[3] = -2 * sa.ih[2]; }
         }
         any = fc[0] != 0.0 || fc[1] != 0.0 || fc[2] != 0.0 || fc[3] != 0.0;
+        xfolds(sa, j, k, xlo, xhi);
+    }
+    // the x-face terms alone (added at the first / last unknown slot of the row)
+    __device__ __forceinline__ static void xfolds(const SolveArgs &sa, int j, int k, double &xlo, double &xhi)
+    {
+        const long long fx = sa.NX + 1, fy = sa.NY + 1;
+        const double *frow = sa.f + j * fx + k * fx * fy;
+        const int bx = sa.bc[0];
         xlo = 0.0;
         xhi = 0.0;
         if (bx == 1 || bx == 2) xlo = -frow[0] * sa.ih2[0];
         else if (bx == 3 || bx == 4) xlo = 2 * sa.bcl[k * fy + j] * sa.ih[0];
         if (bx == 1 || bx == 4) xhi = -frow[sa.NX] * sa.ih2[0];
         else if (bx == 2 || bx == 3) xhi = -2 * sa.bcr[k * fy + j] * sa.ih[0];
+    }
+    // does row (j, k) carry a y or z face fold (the value of any)
+    __device__ __forceinline__ static bool has_faces(const SolveArgs &sa, int j, int k)
+    {
+        return (sa.bc[1] != 0 && (j == sa.s[1] || j == sa.e[1])) || (sa.bc[2] != 0 && (k == sa.s[2] || k == sa.e[2]));
     }
     // the folded rhs at slot s (xs, xe: the unknown x window) from the raw value v = f[row + s]
     __device__ __forceinline__ double apply(double v, int s, int xs, int xe) const

@@ -176,7 +176,7 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         // rows of 32+ lanes (a power of two): the x neighbours of the interior
         // chunks come from the adjacent lanes (below); narrower rows load them
         constexpr bool XSH = (TPL & (TPL - 1)) == 0 && TPL >= 32 && TPL <= 64;
-        constexpr int RC = XSH ? 2 : 4;   // chunks whose loads are issued together
+        constexpr int RC = 2;   // chunks per load group (two groups in flight)
         const int r = t / TPL, l = t % TPL;
         const bool rok = r < nlines;
         const int j = a_tile + (rok ? r : 0);
@@ -204,7 +204,8 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
             else if ((bc == 2 || bc == 3) && hi > lo) { dp[d] = -st[d]; wp[d] = 1.0; }
         }
         const UT *uym = ul + dm[0], *uyp = ul + dp[0], *uzm = ul + dm[1], *uzp = ul + dp[1];
-        const RowFolds rf(sa, j, k);
+        double xlo, xhi;
+        RowFolds::xfolds(sa, j, k, xlo, xhi);
         const int xs = sa.s[0], xe = sa.e[0], bx = sa.bc[0];
         const int xm_end = bx == 0 ? xe : ((bx == 3 || bx == 4) && xe > xs ? xs + 1 : xs);
         const double xm_w = (bx == 0 || ((bx == 3 || bx == 4) && xe > xs)) ? 1.0 : 0.0;
@@ -213,12 +214,13 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         const double pert = sa.pertrb ? sa.pertrb[0] * sa.pert_scale : 0.0;
         const double lam = sa.lambda, ix = sa.ih2[0], iy = sa.ih2[1], iz = sa.ih2[2];
         T *lp = lds_base(buf, r, l);
-        // XSH: U at this lane's slot of every chunk is loaded first; in the
-        // interior chunks the x neighbours are the adjacent lanes' values (the
-        // lanes of a row are one power-of-two segment of the wave), across a
-        // chunk edge the neighbouring chunk's end lane (every window contains
-        // slots 2..N-2 and xe >= N-1, so those values are the plain U of the slot)
-        double u0a[XSH ? NCH : 1];
+        // XSH: U at this lane's slot of every chunk is loaded first (in U's
+        // own type); in the interior chunks the x neighbours are the adjacent
+        // lanes' values (the lanes of a row are one power-of-two segment of
+        // the wave), across a chunk edge the neighbouring chunk's end lane
+        // (every window contains slots 2..N-2 and xe >= N-1, so those values
+        // are the plain U of the slot)
+        UT u0a[XSH ? NCH : 1];
         if constexpr (XSH) {
             static_for<0, NCH>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
@@ -230,72 +232,123 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
                 }
             });
         }
-        static_for<0, (NCH + RC - 1) / RC>([&](auto gc) {
-            constexpr int g = decltype(gc)::value;
-            double fv[RC], u0[RC], um[RC], up[RC], uy[RC], uz[RC];
-            static_for<0, RC>([&](auto ic) {
-                constexpr int ii = decltype(ic)::value, i = g * RC + ii;
-                if constexpr (i < NCH) {
-                    if constexpr (interior(i)) {
-                        fv[ii] = fl[i * TPL];
-                        if constexpr (XSH) {
-                            u0[ii] = u0a[i];
-                            const double nx = __shfl(u0a[i], l + 1, TPL), nx1 = __shfl(u0a[i + 1], l + 1, TPL);
-                            const double pv = __shfl(u0a[i], l - 1, TPL), pv1 = __shfl(u0a[i - 1], l - 1, TPL);
-                            up[ii] = l == TPL - 1 ? nx1 : nx;
-                            um[ii] = l == 0 ? pv1 : pv;
-                        } else {
-                            u0[ii] = ul[i * TPL];
-                            um[ii] = ul[i * TPL - 1];
-                            up[ii] = ul[i * TPL + 1];
-                        }
-                        uy[ii] = wm[0] * uym[i * TPL] + wp[0] * uyp[i * TPL];
-                        uz[ii] = wm[1] * uzm[i * TPL] + wp[1] * uzp[i * TPL];
-                    } else {
-                        const int s = l + i * TPL;
-                        const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
-                        const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
-                        fv[ii] = fr[sc_];
-                        if constexpr (XSH) u0[ii] = u0a[i];
-                        else u0[ii] = ur[sc_];
-                        um[ii] = ur[sm];
-                        up[ii] = ur[sp];
-                        uy[ii] = wm[0] * (ur + dm[0])[sc_] + wp[0] * (ur + dp[0])[sc_];
-                        uz[ii] = wm[1] * (ur + dm[1])[sc_] + wp[1] * (ur + dp[1])[sc_];
-                    }
-                }
-            });
-            if (rf.any) {
+        // the residual of chunk i from its loaded values (f with folds, U and
+        // its neighbours) into the tile
+        auto finish = [&](auto ic, double fvi, double a, double m_, double p_, double uyi, double uzi) {
+            constexpr int i = decltype(ic)::value;
+            const int s = l + i * TPL;
+            double acc = fvi - pert;
+            bool ok = rok;
+            if constexpr (!interior(i)) {
+                acc += (s == xs ? xlo : 0.0) + (s == xe ? xhi : 0.0);
+                m_ *= s > xs ? 1.0 : xm_w;
+                p_ *= s < xe ? 1.0 : xp_w;
+                ok = ok && s >= xs && s <= xe;
+            }
+            double au = lam * a + (m_ - 2 * a + p_) * ix;
+            au += (uyi - 2 * a) * iy;
+            au += (uzi - 2 * a) * iz;
+            if (s < NS) lp[i * lds_step] = ok ? (T)(acc - au) : (T)0;
+        };
+        // every row without its y / z face folds: branch-free, software-
+        // pipelined over chunk groups of RC (the loads of group g + 1 are
+        // issued before the arithmetic of group g; a scheduling barrier after
+        // each group keeps the compiler from hoisting every group's loads,
+        // which would spill)
+        {
+            constexpr int NG = (NCH + RC - 1) / RC;
+            double fq[2][RC];
+            UT yq[2][RC][2], zq[2][RC][2], xq[2][RC][2], aq[2][RC];
+            auto load = [&](auto gc) {
+                constexpr int g = decltype(gc)::value, h = g & 1;
                 static_for<0, RC>([&](auto ic) {
                     constexpr int ii = decltype(ic)::value, i = g * RC + ii;
                     if constexpr (i < NCH) {
-                        const int s = l + i * TPL, sc_ = (s >= xs && s <= xe) ? s : xs;
-#pragma unroll
-                        for (int m = 0; m < 4; m++)
-                            if (rf.fc[m] != 0.0) fv[ii] += rf.fc[m] * rf.fp[m][sc_];
+                        if constexpr (interior(i)) {
+                            fq[h][ii] = fl[i * TPL];
+                            if constexpr (!XSH) {
+                                aq[h][ii] = ul[i * TPL];
+                                xq[h][ii][0] = ul[i * TPL - 1];
+                                xq[h][ii][1] = ul[i * TPL + 1];
+                            }
+                            yq[h][ii][0] = uym[i * TPL];
+                            yq[h][ii][1] = uyp[i * TPL];
+                            zq[h][ii][0] = uzm[i * TPL];
+                            zq[h][ii][1] = uzp[i * TPL];
+                        } else {
+                            const int s = l + i * TPL;
+                            const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
+                            const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
+                            fq[h][ii] = fr[sc_];
+                            if constexpr (!XSH) aq[h][ii] = ur[sc_];
+                            xq[h][ii][0] = ur[sm];
+                            xq[h][ii][1] = ur[sp];
+                            yq[h][ii][0] = (ur + dm[0])[sc_];
+                            yq[h][ii][1] = (ur + dp[0])[sc_];
+                            zq[h][ii][0] = (ur + dm[1])[sc_];
+                            zq[h][ii][1] = (ur + dp[1])[sc_];
+                        }
                     }
                 });
-            }
-            static_for<0, RC>([&](auto ic) {
-                constexpr int ii = decltype(ic)::value, i = g * RC + ii;
-                if constexpr (i < NCH) {
-                    const int s = l + i * TPL;
-                    const double a = u0[ii];
-                    double acc = fv[ii] - pert, m_ = um[ii], p_ = up[ii];
-                    bool ok = rok;
-                    if constexpr (!interior(i)) {
-                        acc += (s == xs ? rf.xlo : 0.0) + (s == xe ? rf.xhi : 0.0);
-                        m_ *= s > xs ? 1.0 : xm_w;
-                        p_ *= s < xe ? 1.0 : xp_w;
-                        ok = ok && s >= xs && s <= xe;
+            };
+            load(std::integral_constant<int, 0>());
+            static_for<0, NG>([&](auto gc) {
+                constexpr int g = decltype(gc)::value, h = g & 1;
+                if constexpr (g + 1 < NG) load(std::integral_constant<int, g + 1>());
+                static_for<0, RC>([&](auto ic) {
+                    constexpr int ii = decltype(ic)::value, i = g * RC + ii;
+                    if constexpr (i < NCH) {
+                        double a, m_, p_;
+                        if constexpr (interior(i) && XSH) {
+                            a = (double)u0a[i];
+                            const UT nx = __shfl(u0a[i], l + 1, TPL), nx1 = __shfl(u0a[i + 1], l + 1, TPL);
+                            const UT pv = __shfl(u0a[i], l - 1, TPL), pv1 = __shfl(u0a[i - 1], l - 1, TPL);
+                            p_ = (double)(l == TPL - 1 ? nx1 : nx);
+                            m_ = (double)(l == 0 ? pv1 : pv);
+                        } else {
+                            if constexpr (XSH) a = (double)u0a[i];
+                            else a = (double)aq[h][ii];
+                            m_ = (double)xq[h][ii][0];
+                            p_ = (double)xq[h][ii][1];
+                        }
+                        const double uyi = wm[0] * (double)yq[h][ii][0] + wp[0] * (double)yq[h][ii][1];
+                        const double uzi = wm[1] * (double)zq[h][ii][0] + wp[1] * (double)zq[h][ii][1];
+                        finish(std::integral_constant<int, i>(), fq[h][ii], a, m_, p_, uyi, uzi);
                     }
-                    double au = lam * a + (m_ - 2 * a + p_) * ix;
-                    au += (uy[ii] - 2 * a) * iy;
-                    au += (uz[ii] - 2 * a) * iz;
-                    if (s < NS) lp[i * lds_step] = ok ? (T)(acc - au) : (T)0;
-                }
+                });
+                __builtin_amdgcn_sched_barrier(0);
             });
-        });
+        }
+        // rows on a y or z face of the unknown box (a few per plane, or the
+        // whole plane at a z face): the residual again with the face folds
+        // (loads from a face array or f), over the lane's own slots, in a
+        // compact loop; the lane wrote these slots above, so no barrier
+        if (RowFolds::has_faces(sa, j, k)) {
+            const RowFolds rf(sa, j, k);
+#pragma unroll 1
+            for (int i = 0; i < NCH; i++) {
+                const int s = l + i * TPL;
+                if (s >= NS) break;
+                const int sc_ = (rok && s >= xs && s <= xe) ? s : xs;
+                const int sm = sc_ > xs ? sc_ - 1 : xm_end, sp = sc_ < xe ? sc_ + 1 : xp_end;
+                double acc = fr[sc_] - pert;
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    if (rf.fc[m] != 0.0) acc += rf.fc[m] * rf.fp[m][sc_];
+                const double a = ur[sc_];
+                double m_ = ur[sm], p_ = ur[sp];
+                acc += (s == xs ? xlo : 0.0) + (s == xe ? xhi : 0.0);
+                m_ *= s > xs ? 1.0 : xm_w;
+                p_ *= s < xe ? 1.0 : xp_w;
+                const bool ok = rok && s >= xs && s <= xe;
+                const double uyi = wm[0] * (double)(ur + dm[0])[sc_] + wp[0] * (double)(ur + dp[0])[sc_];
+                const double uzi = wm[1] * (double)(ur + dm[1])[sc_] + wp[1] * (double)(ur + dp[1])[sc_];
+                double au = lam * a + (m_ - 2 * a + p_) * ix;
+                au += (uyi - 2 * a) * iy;
+                au += (uzi - 2 * a) * iz;
+                lp[i * lds_step] = ok ? (T)(acc - au) : (T)0;
+            }
+        }
     }
 
     // ---- store ---------------------------------------------------------------
@@ -312,16 +365,30 @@ template <typename T, int N, int TPL, int LINES, bool SW> struct Io {
         const int hi = p.st_hi < p.d_rowlen - 1 ? p.st_hi : p.d_rowlen - 1;
         const unsigned lane = (unsigned)(r * (int)p.d_sa + l);
         UOT *orow = out + obase + lane;
-        const UIT *irow = uin ? uin + obase + lane : nullptr;
         const T *lp = lds_base(buf, r, l);
+        // every slot of the iterate is loaded before the first store: out may
+        // be uin (an in-place update; each lane reads and writes only its own
+        // slots), and a store between the loads would make each chunk wait
+        // for the previous one's store to complete
+        double ui[NCH];
+        if (uin) {
+            const UIT *irow = uin + obase + lane;
+#pragma unroll
+            for (int i = 0; i < NCH; i++) {
+                const int s = l + i * TPL;
+                const bool ok = s < NS && (interior(i) || (s >= p.st_lo && s <= hi));
+                ui[i] = ok ? (double)irow[i * TPL] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NCH; i++) ui[i] = 0.0;
+        }
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
             const int s = l + i * TPL;
             if (s >= NS) continue;
             if (!interior(i) && !(s >= p.st_lo && s <= hi)) continue;
-            double v = (double)lp[i * lds_step] * sc;
-            if (irow) v += (double)irow[i * TPL];
-            orow[i * TPL] = (UOT)v;
+            orow[i * TPL] = (UOT)((double)lp[i * lds_step] * sc + ui[i]);
         }
     }
     // slots [st_lo, st_hi] of every line, value * scale inside [out_lo, out_hi], 0 outside
