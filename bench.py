@@ -60,9 +60,9 @@ def log(*a):
 
 def measured_traffic(pass_name):
     """(HBM bytes per launch, kernel name) of the kernel behind a solve pass,
-    from profiles/traffic_512.txt (two rocprofv3 --pmc runs, FETCH_SIZE then
-    WRITE_SIZE, calibrated against a copy kernel of known size); (None, None)
-    if not profiled."""
+    from profiles/traffic_512.txt (written by python -m cuhelmholtz_amd.profsum
+    from two rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, calibrated
+    against a copy kernel of known size); (None, None) if not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic_512.txt")
     if not os.path.exists(path):
         return None, None
@@ -70,9 +70,9 @@ def measured_traffic(pass_name):
         for line in f:
             if line.startswith("#") or not line.strip():
                 continue
-            parts = line.split(None, 3)
+            parts = line.split(None, 5)   # pass, read GB, write GB, average us, launches, kernel
             if parts[0] == pass_name:
-                return (float(parts[1]) + float(parts[2])) * 1e9, parts[3].strip()
+                return (float(parts[1]) + float(parts[2])) * 1e9, parts[5].strip()
     return None, None
 
 

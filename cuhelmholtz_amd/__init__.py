@@ -386,7 +386,7 @@ def get_timing(max_passes=4096):
 
 class BatchSolver:
     """Batched solve of many independent problems (chz_batch_create): one
-    hipGraph per plan, one persistent launch per pass and size class.
+    hipGraph per plan, one launch per pass and size class.
 
     problems: sequence of (f, n, bc, lam[, bcs[, dom]]) with f a contiguous
     CUDA float64 node tensor shaped (NZ+1, NY+1, NX+1) -- bound to the plan:

@@ -30,7 +30,7 @@ class HelmholtzSolve(torch.autograd.Function):
             raise ValueError("lam must be a scalar or hold one value per problem")
         u = fb.clone()
         if batched:
-            # one batched solve (persistent launches, the library's cached plan for this geometry)
+            # one batched solve (one graph replay of the library's cached plan for this geometry)
             C.helmholtz_solve_batch(u, (NX, NY, NZ), bc, lams, [None if b is None else b.detach() for b in bcs], dom)
         else:
             faces = [None if b is None else b.detach().contiguous() for b in bcs]

@@ -114,7 +114,7 @@ static int adjoint(const double dom[6], const int N[3], const int bc[3], double 
 // consecutive node arrays u, gu, gf, per-problem face gradient arrays and
 // lambda gradients.  The node-array steps run once over the whole batch
 // (blockIdx.y = problem); the solve is the batched solve of the library's
-// plan for this geometry (one graph replay, one persistent launch per pass).
+// plan for this geometry (one graph replay, one launch per pass and size class).
 static int adjoint_batch(const double dom[6], const int N[3], const int bc[3], int count, const double *lambdas,
                          const double *u, const double *gu, double *gf, double *const gfaces[6], double *glambda,
                          hipStream_t st)

@@ -154,6 +154,15 @@ static int enqueue(Context &ctx, const BatchPlan &p, hipStream_t st, bool timed)
     return OK;
 }
 
+// The launch configuration of a pass (problems of every kind share a launch:
+// splitting the groups by kind, to run kernels that hold one kind's code,
+// turned the 2048-problem workload's 28 launches into 128 and measured
+// slower, FP64 20.8 -> 22.2 ms)
+static LaunchKey launch_key(const PassSpec &q)
+{
+    return std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+}
+
 static ProbKey key_of(const SolveGeom &g, const chz_problem &q)
 {
     ProbKey k{};
@@ -319,7 +328,7 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         for (int i = 0; i < count; i++) {
             if (s >= specs[i].size()) continue;
             const PassSpec &q = specs[i][s];
-            const LaunchKey key = std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan);
+            const LaunchKey key = launch_key(q);
             auto it = idx.find(key);
             if (it == idx.end()) {
                 idx[key] = groups.size();
@@ -554,7 +563,7 @@ static int rebind(BatchPlan &plan, const chz_problem *probs, int count, hipStrea
     for (const GroupRec &gr : plan.groups)
         for (int i : gr.probs) {
             const PassSpec &q = specs[i][gr.step];
-            if (std::make_tuple(q.N, q.ept, q.d.p.TPL, q.d.p.L, q.d.p.plan) != gr.key) {
+            if (launch_key(q) != gr.key) {
                 if (cfg_changed) *cfg_changed = true;
                 return fail(ERR_INVALID, "rebind: launch configurations changed since the plan was made");
             }

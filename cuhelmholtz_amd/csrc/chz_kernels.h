@@ -154,7 +154,8 @@ const FixedEntry *fixed_kernel(int prec, int op, int N, int ept, int tpl, int li
 const FixedEntry *line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan, int ks = -1);
 // fixed-shape batched kernel (arguments as k_batch), nullptr if none
 const void *batch_fixed_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan);
-// batched kernel with lean tile I/O (arguments as k_batch), nullptr if none
+// batched kernel with lean tile I/O (arguments as k_batch; the kind-specialised
+// one if compiled, else the one for every kind), nullptr if none
 // (CHZ_NO_LINE / CHZ_NO_FIXED in the environment: none); usable when every
 // problem of the launch passes line_windows_ok
 const FixedEntry *batch_line_kernel(int prec, int op, int N, int ept, int tpl, int lines, unsigned long long plan,

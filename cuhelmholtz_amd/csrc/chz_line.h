@@ -19,11 +19,43 @@
 
 namespace chz {
 
-// LDS bytes a kernel stages beyond the tile, FFT twiddles and reduction
-// scratch: the fused z pass keeps the z eigenvalues and w^0..w^(M/2)
-template <typename T, int N, int OP> constexpr unsigned line_lds_extra()
+// LDS of a fixed-shape pass: the tile, the FFT twiddles and the reduction
+// scratch, plus (fused z pass) the z eigenvalues and w^0..w^(M/2)
+template <typename T, int N, int TPL, int L, int OP> constexpr long line_lds_core()
 {
-    return OP == B_ZMID ? (unsigned)((N + 2) * sizeof(double) + (N / 4 + 1) * 2 * sizeof(T)) : 0u;
+    return ((long)(N / 2 + 1) * L + N / 2) * 2 * (long)sizeof(T) + (long)TPL * L * (long)sizeof(T) +
+           (OP == B_ZMID ? (long)((N + 2) * sizeof(double) + (N / 4 + 1) * 2 * sizeof(T)) : 0L);
+}
+// Waves per SIMD that LDS of lds bytes per tile leaves room for (160 KB per
+// CU, at most 4): the register budget of the kernel follows from it.
+template <int TPL, int L> constexpr int line_waves(long lds)
+{
+    const long wgs = 163840 / lds, w = wgs * (TPL * L / 64) / 4;
+    return w < 1 ? 1 : w > 4 ? 4 : (int)w;
+}
+// LDS bytes of the fold / unfold tables (w^0..w^(M/2), unless the fused z
+// pass holds them already, and the M + 1 entry n-table)
+template <typename T, int N, int OP> constexpr long line_ftab_bytes()
+{
+    return (long)((OP == B_ZMID ? 0 : N / 4 + 1) + N / 2 + 1) * 2 * (long)sizeof(T);
+}
+// The fold / unfold tables are staged in LDS where that leaves the waves per
+// SIMD unchanged and the tile within a CU's LDS (else they are read from the
+// twiddle table in global memory)
+template <typename T, int N, int TPL, int L, int OP> constexpr bool line_ftab_lds()
+{
+    return line_lds_core<T, N, TPL, L, OP>() + line_ftab_bytes<T, N, OP>() <= 163840 &&
+           line_waves<TPL, L>(line_lds_core<T, N, TPL, L, OP>() + line_ftab_bytes<T, N, OP>()) ==
+               line_waves<TPL, L>(line_lds_core<T, N, TPL, L, OP>());
+}
+// LDS bytes a kernel stages beyond the tile, FFT twiddles and reduction
+// scratch (BT: the batched kernel, which reads the fold / unfold tables from
+// global memory: its small tiles run more workgroups per CU than the wave
+// bound above counts, and the tables measured slower there)
+template <typename T, int N, int OP, int TPL, int L, bool BT = false> constexpr unsigned line_lds_extra()
+{
+    return (unsigned)(line_lds_core<T, N, TPL, L, OP>() - line_lds_core<T, N, TPL, L, B_ROWS>() +
+                      (!BT && line_ftab_lds<T, N, TPL, L, OP>() ? line_ftab_bytes<T, N, OP>() : 0L));
 }
 
 namespace line {
@@ -737,7 +769,8 @@ __host__ __device__ constexpr int zmid_inv_kind(int c) { return c == 0 ? K_C2R :
 // boundary code), so the kernel holds only that kind's code (a kernel with
 // every kind inlined is several hundred KB, and where its hot path lands in
 // the instruction cache changed its speed by half).
-template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL, int KS = -1>
+template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, bool FULL, int KS = -1,
+          bool BT = false>
 __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, const double *u_in, double *u_out, int a_tile,
                                     int b, int nlines, unsigned char *smem, int t, bool tables = true)
 {
@@ -756,7 +789,7 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
                   PLAN == 0x1010ull) {
         // the tile's LDS (FP64 tile, twiddles, reduction scratch, z tables) holds the wave-resident form's
         static_assert(zmid_wave_lds() <= (unsigned)(((N / 2 + 1) * L + N / 2) * 16 + TPL * L * 8) +
-                                             line_lds_extra<double, N, B_ZMID>(),
+                                             line_lds_extra<double, N, B_ZMID, TPL, L>(),
                       "LDS of the wave-resident fused z pass");
         if (kind == K_R2C && kind2 == K_C2R && p.in_lo == 0 && p.in_hi == N - 1 && p.st_lo == 0 &&
             p.st_hi == N - 1 && sa.out2_lo == 0 && sa.out2_hi == N - 1) {
@@ -780,15 +813,27 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     // the FFT twiddles and the z pass's tables (unchanged while a workgroup
     // stays on one problem: tables == false)
     if (tables) stage_twiddles(twl, tw + TwOffsets::fftw(N), M, TPL * L, t);
-    // fused z pass: the z eigenvalues and w^0..w^(M/2) of the divide, staged
-    // with the tile (their loads overlap the tile's instead of following a barrier)
+    // fused z pass: the z eigenvalues and w^0..w^(M/2) of the divide; with
+    // FT the fold / unfold tables (w^0..w^(M/2) and the kind's n-table), all
+    // staged with the tile (their loads overlap the tile's instead of
+    // following a barrier in the middle of the transform)
+    constexpr bool FT = !BT && line_ftab_lds<T, N, TPL, L, OP>();
     double *ezl = reinterpret_cast<double *>(red + TPL * L);
-    V *wkl = reinterpret_cast<V *>(ezl + (N + 2));
+    V *wkl = reinterpret_cast<V *>(ezl + (OP == B_ZMID ? N + 2 : 0));
+    V *ntl = wkl + (M / 2 + 1);
     if constexpr (OP == B_ZMID) if (tables) {
         const double *ezg = sa.eig + (sa.NX + 2) + (sa.NY + 2);
         for (int i = t; i < N + 2; i += TPL * L) ezl[i] = ezg[i];
-        for (int i = t; i <= M / 2; i += TPL * L) wkl[i] = tw[4 * i];
     }
+    if constexpr (OP == B_ZMID || FT) if (tables)
+        for (int i = t; i <= M / 2; i += TPL * L) wkl[i] = tw[4 * i];
+    // the n-table of the pass kind (the fused z pass: of its forward kind;
+    // the inverse kind of every boundary code uses the same one)
+    if constexpr (FT) if (tables && ntab_of(kind) != 0) {
+        const V *src = ntab_of(kind) == 2 ? tw + TwOffsets::rsin(N) : tw;
+        for (int i = t; i <= M; i += TPL * L) ntl[i] = src[i];
+    }
+    const LTabs<V> ltab{wkl, ntl};
     const T *src = reinterpret_cast<const T *>(p.src) + (long long)a_tile * p.s_sa + (long long)b * p.s_sb;
     if constexpr (OP == B_ROWS || OP == B_XUPD) IO::template stage_rows<FULL>(p, buf, nlines, src, t);
     if constexpr (OP == B_COLS || OP == B_ZMID) IO::template stage_cols<FULL>(p, buf, nlines, src, t);
@@ -810,26 +855,34 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
             else periodic_split_divide<EPT, T, TL, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
         } else {
-            line_transform<EPT, T>(kind, tl, u, TPL, M, N, PLAN, tw, twl, red);
-            if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
-            else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
-            __syncthreads();
-            if (last == K_C2R) {
-                fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
-                fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
-            } else {
-                line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tw, twl, red);
-            }
+            auto zt = [&](const auto &tb) {
+                line_transform<EPT, T>(kind, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
+                else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
+                __syncthreads();
+                if (last == K_C2R) {
+                    fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
+                    fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+                } else {
+                    line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                }
+            };
+            if constexpr (FT) zt(ltab);
+            else zt(tw);
         }
         q.out_lo = sa.out2_lo;
         q.out_hi = sa.out2_hi;
     } else {
-        if (last == K_C2R) {
-            fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
-            fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
-        } else {
-            line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tw, twl, red);
-        }
+        auto xt = [&](const auto &tb) {
+            if (last == K_C2R) {
+                fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
+                fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+            } else {
+                line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+            }
+        };
+        if constexpr (FT) xt(ltab);
+        else xt(tw);
     }
     const long long dbase = (long long)a_tile * p.d_sa + (long long)b * p.d_sb;
     T *dst = reinterpret_cast<T *>(p.dst) + dbase;
@@ -850,16 +903,10 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
 
 }  // namespace line
 
-// Waves per SIMD that the LDS of one tile leaves room for (160 KB per CU, at
-// most 4): the register budget of the kernel follows from it.
-
-template <typename T, int N, int TPL, int L, int OP> constexpr int line_occupancy()
+// Waves per SIMD that the LDS of one tile leaves room for
+template <typename T, int N, int TPL, int L, int OP, bool BT = false> constexpr int line_occupancy()
 {
-    constexpr long lds = ((long)(N / 2 + 1) * L + N / 2) * 2 * (long)sizeof(T) + (long)TPL * L * (long)sizeof(T) +
-                         (long)line_lds_extra<T, N, OP>();
-    constexpr long wgs = 163840 / lds;
-    constexpr long w = wgs * (TPL * L / 64) / 4;
-    return w < 1 ? 1 : w > 4 ? 4 : (int)w;
+    return line_waves<TPL, L>(line_lds_core<T, N, TPL, L, B_ROWS>() + (long)line_lds_extra<T, N, OP, TPL, L, BT>());
 }
 
 // Fixed-shape pass kernel with lean tile I/O (same arguments as k_fixed).
@@ -889,8 +936,8 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>())) 
 // with one workgroup per item, a smaller grid walks several) around the tile
 // body of k_line.  Same precondition on the windows of every problem of the
 // launch.
-template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT>
-__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
+template <typename T, int OP, int N, int TPL, int L, unsigned long long PLAN, int EPT, int KS = -1>
+__global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP, true>()))
     k_line_batch(const BatchItem *__restrict__ items, int nitems, const BatchDesc *__restrict__ descs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -926,9 +973,11 @@ __global__ void __launch_bounds__(L *TPL, (line_occupancy<T, N, TPL, L, OP>()))
         const int a_tile = q.a0 + (it.tile % ntiles) * L, b = q.b0 + it.tile / ntiles;
         const int nlines = min(L, q.a0 + q.na - a_tile);
         if (nlines == L)
-            line::run<T, OP, N, TPL, L, PLAN, EPT, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t, tables);
+            line::run<T, OP, N, TPL, L, PLAN, EPT, true, KS, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t,
+                                                                   tables);
         else
-            line::run<T, OP, N, TPL, L, PLAN, EPT, false>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t, tables);
+            line::run<T, OP, N, TPL, L, PLAN, EPT, false, KS, true>(q, d->sa, d->u_in, d->u_out, a_tile, b, nlines, smem, t,
+                                                                    tables);
         __syncthreads();   // the next item reuses the tile memory
     }
 }

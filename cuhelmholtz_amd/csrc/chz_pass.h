@@ -20,6 +20,33 @@
 
 namespace chz {
 
+// The fold / unfold tables of a pass: w^k = (cos, sin)(2 pi k / N) for
+// k <= M/2, and per n <= M either (cos, sin)(n pi / 2N) (DCT_2 / DST_2 /
+// DCT_3 / DST_3) or (1 / sin(n pi / N), 0) (DST_1 / DCT_1).  A pointer to the
+// pass's twiddle table in global memory reads them there (GTabs); LTabs reads
+// copies staged in LDS, where the n-table holds whichever of the two the
+// kind uses.
+template <typename V> struct GTabs {
+    const V *tw;
+    int N;
+    __device__ __forceinline__ V wk(int k) const { return tw[4 * k]; }
+    __device__ __forceinline__ V cs(int n) const { return tw[n]; }
+    __device__ __forceinline__ V rs(int n) const { return tw[TwOffsets::rsin(N) + n]; }
+};
+template <typename V> struct LTabs {
+    const V *w, *n;
+    __device__ __forceinline__ V wk(int k) const { return w[k]; }
+    __device__ __forceinline__ V cs(int i) const { return n[i]; }
+    __device__ __forceinline__ V rs(int i) const { return n[i]; }
+};
+template <typename V> __device__ __forceinline__ GTabs<V> tabs_of(const V *tw, int N) { return GTabs<V>{tw, N}; }
+template <typename V> __device__ __forceinline__ LTabs<V> tabs_of(const LTabs<V> &t, int) { return t; }
+// the n-table a kind uses: 1 = (cos, sin)(n pi / 2N), 2 = 1 / sin(n pi / N), 0 = none
+__host__ __device__ constexpr int ntab_of(int k)
+{
+    return (k == K_DCT_2 || k == K_DST_2 || k == K_DCT_3 || k == K_DST_3) ? 1 : (k == K_DST_1 || k == K_DCT_1) ? 2 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Fold: real line (slots) -> FFT input points 0..M-1, in place.
 // ---------------------------------------------------------------------------
@@ -55,11 +82,11 @@ __device__ __forceinline__ typename vec2_of<T>::type hval(const TL &tl, int k, i
     }
 }
 
-template <int KIND, int EPT, typename T, typename TL>
-__device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N,
-                                     const typename vec2_of<T>::type *__restrict__ tw, T *red)
+template <int KIND, int EPT, typename T, typename TL, typename TB>
+__device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N, const TB &tb, T *red)
 {
     typedef typename vec2_of<T>::type V;
+    const auto tbs = tabs_of(tb, N);
     constexpr int PP = EPT / 2 + 1;        // >= ceil((M/2 + 1)/TPL)
     constexpr int PT = EPT + 1;            // >= ceil((M + 1)/TPL)
     if constexpr (kind_is_z2d(KIND)) {
@@ -76,7 +103,7 @@ __device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N,
 #pragma unroll
         for (int i = 0; i < PP; i++) {
             const int k = u + i * TPL;
-            wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);
+            wk[i] = k <= M / 2 ? tbs.wk(k) : mk<V>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PP; i++) {
@@ -107,7 +134,7 @@ __device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N,
 #pragma unroll
         for (int i = 0; i < PT; i++) {
             const int t = u + i * TPL;
-            csv[i] = t <= M ? tw[t] : mk<V>(0, 0);   // (cos, sin)(t pi / 2N)
+            csv[i] = t <= M ? tbs.cs(t) : mk<V>(0, 0);   // (cos, sin)(t pi / 2N)
         }
 #pragma unroll
         for (int i = 0; i < PT; i++) {
@@ -152,11 +179,11 @@ __device__ __forceinline__ void fold(const TL &tl, int u, int TPL, int M, int N,
 // ---------------------------------------------------------------------------
 // Unfold: FFT output points -> real output slots, in place.
 // ---------------------------------------------------------------------------
-template <int KIND, int EPT, typename T, typename TL>
-__device__ __forceinline__ void unfold(const TL &tl, int u, int TPL, int M, int N,
-                                       const typename vec2_of<T>::type *__restrict__ tw, const T *red)
+template <int KIND, int EPT, typename T, typename TL, typename TB>
+__device__ __forceinline__ void unfold(const TL &tl, int u, int TPL, int M, int N, const TB &tb, const T *red)
 {
     typedef typename vec2_of<T>::type V;
+    const auto tbs = tabs_of(tb, N);
     constexpr int PP = EPT / 2 + 1;
     constexpr int PT = EPT + 1;
     const T h = (T)0.5;
@@ -170,11 +197,12 @@ __device__ __forceinline__ void unfold(const TL &tl, int u, int TPL, int M, int 
         }
         // per-n table value: (cos, sin)(n pi/2N) for DCT_2 / DST_2, (1/sin(n pi/N), 0) for DST_1 / DCT_1
         V tab[PT];
-        const V *tsrc = (KIND == K_DCT_2 || KIND == K_DST_2) ? tw : tw + TwOffsets::rsin(N);
 #pragma unroll
         for (int i = 0; i < PT; i++) {
             const int n = u + i * TPL;
-            tab[i] = (KIND != K_C2R && n <= M) ? tsrc[n] : mk<V>(0, 0);
+            if constexpr (KIND == K_DCT_2 || KIND == K_DST_2) tab[i] = n <= M ? tbs.cs(n) : mk<V>(0, 0);
+            else if constexpr (KIND != K_C2R) tab[i] = n <= M ? tbs.rs(n) : mk<V>(0, 0);
+            else tab[i] = mk<V>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PT; i++) {
@@ -235,7 +263,7 @@ __device__ __forceinline__ void unfold(const TL &tl, int u, int TPL, int M, int 
 #pragma unroll
         for (int i = 0; i < PP; i++) {
             const int k = u + i * TPL;
-            wk[i] = k <= M / 2 ? tw[4 * k] : mk<V>(0, 0);
+            wk[i] = k <= M / 2 ? tbs.wk(k) : mk<V>(0, 0);
         }
 #pragma unroll
         for (int i = 0; i < PP; i++) {
@@ -462,9 +490,9 @@ template <typename T> struct Smem {
 };
 
 // One line transform of the whole tile (fold, FFT, unfold), data in LDS.
-template <int KIND, int EPT, typename T, typename TL>
+template <int KIND, int EPT, typename T, typename TL, typename TB>
 __device__ __forceinline__ void line_transform_k(const TL &tl, int u, int TPL, int M, int N,
-                                                 unsigned long long plan, const typename vec2_of<T>::type *tw,
+                                                 unsigned long long plan, const TB &tw,
                                                  const typename vec2_of<T>::type *twl, T *red)
 {
     fold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
@@ -473,10 +501,11 @@ __device__ __forceinline__ void line_transform_k(const TL &tl, int u, int TPL, i
 }
 
 // Runtime kind -> one specialised path (the branch is uniform over the workgroup).
-// tw: the pass's twiddle table in global memory; twl: its FFT part staged in LDS.
-template <int EPT, typename T, typename TL>
+// tw: the pass's twiddle table in global memory, or the fold / unfold tables
+// staged in LDS (LTabs); twl: the FFT twiddles staged in LDS.
+template <int EPT, typename T, typename TL, typename TB>
 __device__ __forceinline__ void line_transform(int kind, const TL &tl, int u, int TPL, int M, int N,
-                                               unsigned long long plan, const typename vec2_of<T>::type *tw,
+                                               unsigned long long plan, const TB &tw,
                                                const typename vec2_of<T>::type *twl, T *red)
 {
     switch (kind) {

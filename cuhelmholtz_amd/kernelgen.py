@@ -17,7 +17,7 @@ transform's y pass; xfwd: solve pass 1, plus the FP64 residual pass in FP32;
 zmid: the fused z pass; zcols: the transform's z pass):
 
   * an even thread count per line: the lean tile I/O kernel k_line, and for
-    N <= 128 (the batched size classes) its persistent batched form
+    N <= 128 (the batched size classes) its batched form
     k_line_batch, so a batched problem and its single solve run one tile body
     (FP32 batches end with the update pass, so their rows pass is not built);
   * an odd thread count: k_fixed, and for N <= 128 k_batch_fixed.
@@ -194,12 +194,17 @@ def _entry(k):
     if form == F_LINE:
         fn = f"k_line<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}" + (f", {ks}>" if ks >= 0 else ">")
     elif form == F_LINE_BATCH:
-        fn = f"k_line_batch<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}>"
+        fn = f"k_line_batch<{T}, {opn}, {N}, {tpl}, {lines}, {pl}, {ept}" + (f", {ks}>" if ks >= 0 else ">")
     elif form == F_FIXED:
         fn = f"k_fixed<{T}, {ept}, {opn}, {N}, {tpl}, {lines}, {pl}>"
     else:
         fn = f"k_batch_fixed<{T}, {ept}, {opn}, {N}, {tpl}, {lines}, {pl}>"
-    extra = f"line_lds_extra<{T}, {N}, {opn}>()" if form in (F_LINE, F_LINE_BATCH) else "0u"
+    if form == F_LINE:
+        extra = f"line_lds_extra<{T}, {N}, {opn}, {tpl}, {lines}>()"
+    elif form == F_LINE_BATCH:
+        extra = f"line_lds_extra<{T}, {N}, {opn}, {tpl}, {lines}, true>()"
+    else:
+        extra = "0u"
     return (f"    {{{prec}, {opn}, {N}, {ept}, {tpl}, {lines}, {pl}, (const void *){fn},\n"
             f"     {extra}, F_{FORM_NAMES[form].upper()}, {ks}}},")
 

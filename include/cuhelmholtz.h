@@ -170,8 +170,8 @@ typedef struct chz_problem {
  * (chz_batch_refine_counts reports them).  The
  * plan owns one workspace per problem, device tables of per-problem pass
  * descriptors and its own copies of the twiddle and eigenvalue tables it
- * reads, and holds the whole solve captured as one hipGraph: one
- * persistent launch per pass and size class, then the periodic copies.  The
+ * reads, and holds the whole solve captured as one hipGraph: one launch
+ * per pass and size class, then the periodic copies.  The
  * problems' f arrays and face arrays are bound here: each chz_batch_solve
  * solves whatever they hold at that time.  Results equal the single-problem
  * solve's (same kernels and launch configurations).  No reference
