@@ -483,8 +483,10 @@ class BatchSolver:
             pass
 
 
-ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS = range(5)
-ROLE_NAMES = ["rows", "cols", "xfwd", "zmid", "zcols"]
+# roles of the launch configurations (ROLE_ZMIDN: the fused z pass of a
+# solve with non-periodic z; a size without its own uses the ROLE_ZMID one)
+ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS, ROLE_ZMIDN = range(6)
+ROLE_NAMES = ["rows", "cols", "xfwd", "zmid", "zcols", "zmidn"]
 
 
 def helmholtz_solve_mixed(f, n, bc, lam, bcs=(None,) * 6, dom=(0.0, 1.0) * 3, refine=1):

@@ -138,7 +138,8 @@ def tune_solve(n, prec=64, bc=(3, 4, 0), lam=-4.0, log=print, compiled_only=Fals
     from cuhelmholtz_amd import checks
     from cuhelmholtz_amd import workloads as W
     dev = torch.device("cuda:0")
-    f, bcs = W.smooth_modes(n)
+    # the benchmark rhs for its codes, else N(0,1) data with the face arrays the codes need
+    f, bcs = W.smooth_modes(n) if tuple(bc) == (3, 4, 0) else W.problem_arrays({"n": n, "bc": bc, "seed": 5})
     f0 = torch.from_numpy(f).to(dev)
     bct = [None if b is None else torch.from_numpy(b).to(dev) for b in bcs]
     del f
@@ -164,7 +165,9 @@ def tune_solve(n, prec=64, bc=(3, 4, 0), lam=-4.0, log=print, compiled_only=Fals
         return True, ""
 
     table = []
-    plan = [(C.ROLE_COLS, n[1], ("yfwd", "yinv")), (C.ROLE_ZMID, n[2], ("zmid",)),
+    # the fused z pass has a role per z periodicity (its transforms differ)
+    zrole = C.ROLE_ZMID if bc[2] == 0 else C.ROLE_ZMIDN
+    plan = [(C.ROLE_COLS, n[1], ("yfwd", "yinv")), (zrole, n[2], ("zmid",)),
             (C.ROLE_XFWD, n[0], ("xfwd",)), (C.ROLE_ROWS, n[0], ("xinv",))]
     for role, N, names in plan:
         if roles is not None and role not in roles:
@@ -354,6 +357,9 @@ def emit_alt(path, rows, k, chosen_paths):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--solve", type=int, nargs=3)
+    ap.add_argument("--bc", type=int, nargs=3, default=(3, 4, 0),
+                    help="boundary codes of the tuned solve (a non-periodic z code tunes role 5)")
+    ap.add_argument("--roles", type=int, nargs="*", help="only these roles of the tuned solve")
     ap.add_argument("--transform", type=int)
     ap.add_argument("--batch", type=int, nargs="*", help="size classes of the batched solve to tune")
     ap.add_argument("--prec", type=int, default=64)
@@ -367,7 +373,8 @@ def main():
     out = args.out or chosen
     rows = []
     if args.solve:
-        rows += tune_solve(tuple(args.solve), args.prec, compiled_only=args.compiled_only)
+        rows += tune_solve(tuple(args.solve), args.prec, bc=tuple(args.bc), compiled_only=args.compiled_only,
+                           roles=args.roles)
     if args.transform:
         rows += tune_transform(args.transform, args.prec, compiled_only=args.compiled_only)
     for N in args.batch or []:
@@ -381,7 +388,8 @@ def main():
         return 0
     os.makedirs(os.path.dirname(out), exist_ok=True)
     header = ("# launch configurations chosen by cuhelmholtz_amd.autotune on MI355X (gfx950)\n"
-              "# role N prec ept tpl lines   (role: 0 rows, 1 cols, 2 solve xfwd, 3 solve zmid, 4 z cols)\n")
+              "# role N prec ept tpl lines   (role: 0 rows, 1 cols, 2 solve xfwd, 3 solve zmid (periodic z),\n"
+              "# 4 z cols, 5 solve zmid (z codes 1-4))\n")
     write_table(out, rows, header)
     return 0
 

@@ -204,6 +204,9 @@ bool Context::config(int N, int prec, int role, AxisCfg *out)
         *out = it->second;
         return true;
     }
+    // the non-periodic fused z pass without a configuration of its own: the
+    // periodic one's (not cached, so a later change of that one is followed)
+    if (role == ROLE_ZMIDN) return config(N, prec, ROLE_ZMID, out);
     const int M = N / 2;
     // 8 points per thread first (measured best for FP64 on gfx950), then larger / smaller
     static const int epts[] = {8, 16, 12, 10, 6, 5, 4};

@@ -18,8 +18,11 @@ struct AxisCfg {
     unsigned long long plan = 0;   // radices, one byte per stage
 };
 
-// Kernel roles that keep separate launch configurations.
-enum Role { ROLE_ROWS = 0, ROLE_COLS = 1, ROLE_XFWD = 2, ROLE_ZMID = 3, ROLE_ZCOLS = 4, ROLE_COUNT = 5 };
+// Kernel roles that keep separate launch configurations.  ROLE_ZMIDN: the
+// fused z pass of a solve whose z code is not periodic (its transforms are
+// DCT / DST kinds, which want other tile shapes than the periodic pass); a
+// size with no configuration of its own for it uses the ROLE_ZMID one.
+enum Role { ROLE_ROWS = 0, ROLE_COLS = 1, ROLE_XFWD = 2, ROLE_ZMID = 3, ROLE_ZCOLS = 4, ROLE_ZMIDN = 5, ROLE_COUNT = 6 };
 
 // Status codes of the C ABI (also in include/cuhelmholtz.h).
 enum Status { OK = 0, ERR_INVALID = -1, ERR_UNSUPPORTED = -2, ERR_DEVICE = -3, ERR_ALLOC = -4 };

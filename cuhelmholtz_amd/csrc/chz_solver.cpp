@@ -191,7 +191,8 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
     const int prec = sizeof(T) == 8 ? 64 : 32;
     const int NX = g.N[0], NY = g.N[1], NZ = g.N[2];
     AxisCfg cx, cy, cz;
-    if (!ctx.config(NX, prec, ROLE_XFWD, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) || !ctx.config(NZ, prec, ROLE_ZMID, &cz))
+    const int zrole = g.bc[2] == 0 ? ROLE_ZMID : ROLE_ZMIDN;
+    if (!ctx.config(NX, prec, ROLE_XFWD, &cx) || !ctx.config(NY, prec, ROLE_COLS, &cy) || !ctx.config(NZ, prec, zrole, &cz))
         return fail(ERR_UNSUPPORTED, "no launch configuration for this cell count (LDS or thread limits)");
     const void *twx = tabs ? tabs->tw[0] : ctx.twiddles(NX, prec), *twy = tabs ? tabs->tw[1] : ctx.twiddles(NY, prec),
                *twz = tabs ? tabs->tw[2] : ctx.twiddles(NZ, prec);
@@ -256,7 +257,7 @@ int solve_passes(Context &ctx, const SolveGeom &g, double *f, T *ex, const doubl
     // pass 3: forward z, divide, inverse z
     {
         PassSpec q{};
-        q.op = B_ZMID; q.prec = prec; q.ept = cz.EPT; q.N = NZ; q.role = ROLE_ZMID; q.name = "zmid";
+        q.op = B_ZMID; q.prec = prec; q.ept = cz.EPT; q.N = NZ; q.role = zrole; q.name = "zmid";
         PassArgs &p = q.d.p;
         fill_pass(p, cz, prec, g.fwd[2], twz);
         p.src = ex; p.dst = ex;

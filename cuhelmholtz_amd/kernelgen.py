@@ -45,7 +45,8 @@ TUNING = os.path.join(HERE, "tuning")
 
 B_ROWS, B_COLS, B_XFWD, B_ZMID, B_XRES, B_XUPD = range(6)
 OP_NAMES = ["B_ROWS", "B_COLS", "B_XFWD", "B_ZMID", "B_XRES", "B_XUPD"]
-ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS = range(5)
+# roles of the tuning tables (ROLE_ZMIDN: the fused z pass for non-periodic z)
+ROLE_ROWS, ROLE_COLS, ROLE_XFWD, ROLE_ZMID, ROLE_ZCOLS, ROLE_ZMIDN = range(6)
 # forms of a compiled shape (FixedEntry::form)
 F_LINE, F_LINE_BATCH, F_FIXED, F_FIXED_BATCH = range(4)
 FORM_NAMES = ["line", "line_batch", "fixed", "fixed_batch"]
@@ -119,7 +120,7 @@ def axis_pass_lds(prec, M, L, tpl):
 def role_ops(role, prec):
     """The passes (BatchOp) that run with a role's launch configuration."""
     ops = {ROLE_ROWS: [B_ROWS], ROLE_COLS: [B_COLS], ROLE_XFWD: [B_XFWD], ROLE_ZMID: [B_ZMID],
-           ROLE_ZCOLS: [B_COLS]}[role]
+           ROLE_ZCOLS: [B_COLS], ROLE_ZMIDN: [B_ZMID]}[role]
     if prec == 32 and role == ROLE_ROWS:
         ops = ops + [B_XUPD]
     if prec == 32 and role == ROLE_XFWD:
@@ -148,7 +149,7 @@ def check_row(row, where=""):
     checks of Context::set_config)."""
     role, N, prec, ept, tpl, lines = row
     M = N // 2
-    if role not in range(5) or prec not in (32, 64) or ept not in EPTS or N < 2 or N % 2:
+    if role not in range(6) or prec not in (32, 64) or ept not in EPTS or N < 2 or N % 2:
         raise ValueError(f"{where}invalid tuning row {row}")
     if tpl < 1 or lines < 1 or tpl * lines > MAX_THREADS:
         raise ValueError(f"{where}tuning row {row}: threads per tile out of range")
@@ -169,6 +170,8 @@ def kernel_set(paths=None, arch="gfx950"):
     lines, plan, ks), ks = -1 for every kind, else the kind selector."""
     out = set()
     paths = paths or table_paths(arch)
+    # sizes whose non-periodic fused z pass has a chosen shape of its own
+    zmidn = {(r[1], r[2]) for r in read_table(paths[0]) if r[0] == ROLE_ZMIDN}
     for pi, path in enumerate(paths):
         for row in read_table(path):
             role, N, prec, ept, tpl, lines = row
@@ -179,9 +182,13 @@ def kernel_set(paths=None, arch="gfx950"):
                 # batched solves end their FP32 passes with the update pass, not a rows pass
                 if N <= BATCH_MAX_N and not (prec == 32 and op == B_ROWS):
                     out.add((batched, prec, op, N, ept, tpl, lines, plan, -1))
-                # kind-specialised forms of the chosen large shapes (first table only)
-                if pi == 0 and N in SPECIALIZE_N and single == F_LINE:
-                    for ks in kind_sels(op):
+                # kind-specialised forms of the chosen large shapes (first table
+                # only); the non-periodic fused z pass runs the kernel for every
+                # kind (its per-code FP32 kernels spilled), so where it has a row
+                # the periodic role's shape is specialised for code 0 only
+                if pi == 0 and N in SPECIALIZE_N and single == F_LINE and role != ROLE_ZMIDN:
+                    sels = (0,) if role == ROLE_ZMID and (N, prec) in zmidn else kind_sels(op)
+                    for ks in sels:
                         out.add((single, prec, op, N, ept, tpl, lines, plan, ks))
     return sorted(out)
 

@@ -224,8 +224,10 @@ void chz_release(void);
  * threads per line, lines per workgroup and the radix plan (one byte per
  * stage).  prec = 64 or 32; role 0 = x-axis rows, 1 = y columns (and the
  * solve's y passes), 2 = first solve pass (folds + forward x), 3 = fused z
- * pass of the solve, 4 = z columns of do_transform.  No reference
- * counterpart (the reference's cuFFT plans are opaque). */
+ * pass of the solve with periodic z, 4 = z columns of do_transform, 5 = fused
+ * z pass of the solve with z codes 1-4 (a size without a role-5
+ * configuration of its own uses role 3's).  No reference counterpart (the
+ * reference's cuFFT plans are opaque). */
 int chz_axis_config(int N, int prec, int role, int *ept, int *tpl, int *lines,
                     unsigned long long *plan);
 

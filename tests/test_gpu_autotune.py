@@ -38,11 +38,12 @@ def tuned_state():
     C.load_tuning()
 
 
-def _kernel_kinds(n, prec):
+def _kernel_kinds(n, prec, bc):
     """per role: is the current configuration of (N, prec, role) compiled?"""
     have = {(k[1], k[2], k[3], k[4], k[5], k[6]) for k in C.compiled_kernels() if k[0] in (G.F_LINE, G.F_FIXED)}
     out = []
-    for role, N in ((C.ROLE_XFWD, n[0]), (C.ROLE_COLS, n[1]), (C.ROLE_ZMID, n[2]), (C.ROLE_ROWS, n[0])):
+    zrole = C.ROLE_ZMID if bc[2] == 0 else C.ROLE_ZMIDN
+    for role, N in ((C.ROLE_XFWD, n[0]), (C.ROLE_COLS, n[1]), (zrole, n[2]), (C.ROLE_ROWS, n[0])):
         cfg = C.axis_config(N, prec, role=role)
         out.append(all((prec, op, N, cfg["ept"], cfg["tpl"], cfg["lines"]) in have for op in G.role_ops(role, prec)))
     return out
@@ -64,10 +65,10 @@ def test_tuned_and_builtin_match_oracle(tuned_state, n, bc, lam):
         if label == "tuned":
             C.reset_axis_configs()
             C.load_tuning()
-            assert all(_kernel_kinds(n, 64)), "tuned configurations run compiled kernels"
+            assert all(_kernel_kinds(n, 64, bc)), "tuned configurations run compiled kernels"
         else:
             C.reset_axis_configs()
-            assert not all(_kernel_kinds(n, 64)), "the built-in choice differs somewhere"
+            assert not all(_kernel_kinds(n, 64, bc)), "the built-in choice differs somewhere"
         u = torch.from_numpy(f).to(dev)
         C.helmholtz_solve(u, n, bc, lam, bct)
         m = torch.from_numpy(f).to(dev)

@@ -118,22 +118,27 @@ def test_long_axis_every_bc_vs_oracle(axis):
             assert rel(ft.cpu().numpy(), O.helmholtz(f, n, bc, lam, bcs, stencil=19)) < TOL, (n, bc, lam, 19)
 
 
+ZMID_512_CASES = [((18, 6, 512), (3, 1, 0), -2.3), ((32, 8, 512), (0, 0, 0), 0.0), ((32, 8, 512), (0, 4, 0), 0.5),
+                  ((18, 6, 512), (2, 3, 4), -1.0), ((10, 6, 512), (4, 2, 1), -7.0), ((32, 6, 512), (1, 0, 3), 0.0),
+                  ((16, 6, 512), (3, 4, 2), -3.1)]
+
+
 def test_zmid_512_every_compiled_shape_vs_oracle():
     """Every compiled launch shape of the 512-cell fused z pass (FP64), the
-    wave-resident periodic form (16 x 16 points per line) among them, on z
+    wave-resident periodic form (16 x 16 points per line) among them, as the
+    configuration of both fused-z roles (periodic z and z codes 1-4), on z
     lines of every code, full and partial 16-column tiles (NX = 18 with a
     19-column DCT-I window, NX = 32 periodic), the singular periodic /
     Neumann case and the 19-point operator, against the oracle."""
     from cuhelmholtz_amd import autotune as A
     shapes = A.compiled_shapes(512, 64, C.ROLE_ZMID)
-    assert (16, 16, 16) in shapes
+    assert (16, 16, 16) in shapes and (8, 32, 16) in shapes
     rng = np.random.default_rng(77)
-    cases = [((18, 6, 512), (3, 1, 0), -2.3), ((32, 8, 512), (0, 0, 0), 0.0), ((32, 8, 512), (0, 4, 0), 0.5),
-             ((18, 6, 512), (2, 3, 4), -1.0), ((10, 6, 512), (4, 2, 1), -7.0), ((32, 6, 512), (1, 0, 3), 0.0)]
     try:
         for ept, tpl, lines in shapes:
-            C.set_axis_config(512, 64, C.ROLE_ZMID, ept, tpl, lines)
-            for n, bc, lam in cases:
+            for role in (C.ROLE_ZMID, C.ROLE_ZMIDN):
+                C.set_axis_config(512, 64, role, ept, tpl, lines)
+            for n, bc, lam in ZMID_512_CASES:
                 f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
                 bcs = faces(rng, n, bc)
                 for stencil in (7, 19):
@@ -142,6 +147,55 @@ def test_zmid_512_every_compiled_shape_vs_oracle():
                     torch.cuda.synchronize()
                     want = O.helmholtz(f, n, bc, lam, bcs, stencil=stencil)
                     assert rel(ft.cpu().numpy(), want) < TOL, ((ept, tpl, lines), n, bc, lam, stencil)
+    finally:
+        C.reset_axis_configs()
+        C.load_tuning()
+
+
+def test_zmid_512_fp32_every_compiled_shape_vs_oracle():
+    """The same for the FP32 fused z pass of the mixed solve (two FP64
+    refinements: some of the cases have spectral condition numbers above 1e5,
+    where one leaves a few 1e-10; bar 1e-10 against the oracle)."""
+    from cuhelmholtz_amd import autotune as A
+    shapes = A.compiled_shapes(512, 32, C.ROLE_ZMID)
+    assert (8, 32, 16) in shapes
+    rng = np.random.default_rng(78)
+    try:
+        for ept, tpl, lines in shapes:
+            for role in (C.ROLE_ZMID, C.ROLE_ZMIDN):
+                C.set_axis_config(512, 32, role, ept, tpl, lines)
+            for n, bc, lam in ZMID_512_CASES:
+                f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+                bcs = faces(rng, n, bc)
+                ft = torch.from_numpy(f.copy()).to(dev())
+                C.helmholtz_solve_mixed(ft, n, bc, lam, bcs, refine=2)
+                torch.cuda.synchronize()
+                want = O.helmholtz(f, n, bc, lam, bcs)
+                assert rel(ft.cpu().numpy(), want) < 1e-10, ((ept, tpl, lines), n, bc, lam)
+    finally:
+        C.reset_axis_configs()
+        C.load_tuning()
+
+
+def test_zmid_roles():
+    """The non-periodic fused z pass (role 5) uses the periodic one's
+    configuration (role 3) until it has one of its own, also after role 3
+    changes; a role-5 configuration is then its own, and a solve with
+    non-periodic z runs with it (parity against the oracle)."""
+    rng = np.random.default_rng(79)
+    try:
+        C.set_axis_config(64, 64, C.ROLE_ZMID, 8, 4, 16)
+        assert C.axis_config(64, 64, role=C.ROLE_ZMIDN) == C.axis_config(64, 64, role=C.ROLE_ZMID)
+        C.set_axis_config(64, 64, C.ROLE_ZMID, 8, 4, 32)
+        assert C.axis_config(64, 64, role=C.ROLE_ZMIDN)["lines"] == 32
+        C.set_axis_config(64, 64, C.ROLE_ZMIDN, 4, 8, 8)
+        assert C.axis_config(64, 64, role=C.ROLE_ZMIDN)["ept"] == 4
+        assert C.axis_config(64, 64, role=C.ROLE_ZMID)["lines"] == 32
+        for bc in ((3, 4, 0), (3, 4, 2)):
+            n = (24, 16, 64)
+            f = rng.standard_normal((n[2] + 1, n[1] + 1, n[0] + 1))
+            bcs = faces(rng, n, bc)
+            assert rel(gpu_solve(f, n, bc, -2.0, bcs), O.helmholtz(f, n, bc, -2.0, bcs)) < TOL, bc
     finally:
         C.reset_axis_configs()
         C.load_tuning()
