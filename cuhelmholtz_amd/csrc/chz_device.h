@@ -203,12 +203,15 @@ template <typename T> struct Tile {
 
 // One Stockham stage of radix R over M points, in place.  Thread u of a line
 // runs the butterflies b = u + i*TPL (i < EPT/R, b < M/R), so the stage fits
-// when ceil(M/(R*TPL)) <= EPT/R (the host planner guarantees it).
-template <int R, int EPT, typename TL>
-__device__ __forceinline__ void fft_stage(const TL &tl, int u, int TPL, int M, int Ns, const typename TL::V *__restrict__ tw)
+// when ceil(M/(R*TPL)) <= EPT/R (the host planner guarantees it).  NSC > 0:
+// the span Ns as a compile-time constant (b mod Ns and the twiddle stride
+// then need no integer division).
+template <int R, int EPT, int NSC = 0, typename TL>
+__device__ __forceinline__ void fft_stage(const TL &tl, int u, int TPL, int M, int Ns_rt, const typename TL::V *__restrict__ tw)
 {
     typedef typename TL::V V;
     constexpr int NB = EPT / R;
+    const int Ns = NSC > 0 ? NSC : Ns_rt;
     const int MR = M / R;
     V v[NB * R];
 #pragma unroll
@@ -331,6 +334,26 @@ __device__ __forceinline__ void fft_forward(const TL &tl, int u, int TPL, int M,
                 break;
         }
         Ns *= R;
+    }
+}
+
+// fft_forward with the radix plan a compile-time constant (the fixed-shape
+// kernels): the stages unroll, so every stage's span Ns is a constant and
+// the butterflies' index arithmetic needs no integer division (the loop
+// above keeps Ns a run-time value and divides by it per butterfly).
+template <int EPT, typename T, unsigned long long PLAN, int ST = 0, int NS = 1, typename TL>
+__device__ __forceinline__ void fft_forward_c(const TL &tl, int u, int TPL, int M,
+                                              const typename vec2_of<T>::type *__restrict__ tw)
+{
+    constexpr int R = ST < 8 ? (int)((PLAN >> (8 * ST)) & 255) : 0;
+    if constexpr (R == 1) {
+        // the whole remaining factor, one direct stage
+        fft_stage_any<EPT>(tl, u, TPL, M, NS, M / NS, tw);
+    } else if constexpr (R != 0) {
+        constexpr bool SPEC = R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 || R == 10 || R == 12 || R == 16;
+        if constexpr (SPEC && EPT >= R) fft_stage<R, EPT, NS>(tl, u, TPL, M, NS, tw);
+        else fft_stage_any<EPT>(tl, u, TPL, M, NS, R, tw);
+        fft_forward_c<EPT, T, PLAN, ST + 1, NS * R>(tl, u, TPL, M, tw);
     }
 }
 

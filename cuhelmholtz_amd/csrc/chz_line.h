@@ -850,21 +850,21 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     if constexpr (OP == B_ZMID) {
         last = kind2;
         if (kind == K_R2C && kind2 == K_C2R) {
-            fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+            fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
             if (sa.stencil19) periodic_split_divide<EPT, T, TL, true, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             else periodic_split_divide<EPT, T, TL, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
-            fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+            fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
         } else {
             auto zt = [&](const auto &tb) {
-                line_transform<EPT, T>(kind, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                line_transform<EPT, T, PLAN>(kind, tl, u, TPL, M, N, PLAN, tb, twl, red);
                 if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 __syncthreads();
                 if (last == K_C2R) {
                     fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                    fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+                    fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
                 } else {
-                    line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                    line_transform<EPT, T, PLAN>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
                 }
             };
             if constexpr (FT) zt(ltab);
@@ -876,9 +876,9 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         auto xt = [&](const auto &tb) {
             if (last == K_C2R) {
                 fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+                fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
             } else {
-                line_transform<EPT, T>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                line_transform<EPT, T, PLAN>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
             }
         };
         if constexpr (FT) xt(ltab);

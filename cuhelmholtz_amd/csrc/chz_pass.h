@@ -490,33 +490,35 @@ template <typename T> struct Smem {
 };
 
 // One line transform of the whole tile (fold, FFT, unfold), data in LDS.
-template <int KIND, int EPT, typename T, typename TL, typename TB>
+// PLANC != 0: the radix plan as a compile-time constant (fixed-shape kernels)
+template <int KIND, int EPT, typename T, unsigned long long PLANC = 0, typename TL, typename TB>
 __device__ __forceinline__ void line_transform_k(const TL &tl, int u, int TPL, int M, int N,
                                                  unsigned long long plan, const TB &tw,
                                                  const typename vec2_of<T>::type *twl, T *red)
 {
     fold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
-    fft_forward<EPT, T>(tl, u, TPL, M, plan, twl);
+    if constexpr (PLANC != 0) fft_forward_c<EPT, T, PLANC>(tl, u, TPL, M, twl);
+    else fft_forward<EPT, T>(tl, u, TPL, M, plan, twl);
     unfold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
 }
 
 // Runtime kind -> one specialised path (the branch is uniform over the workgroup).
 // tw: the pass's twiddle table in global memory, or the fold / unfold tables
 // staged in LDS (LTabs); twl: the FFT twiddles staged in LDS.
-template <int EPT, typename T, typename TL, typename TB>
+template <int EPT, typename T, unsigned long long PLANC = 0, typename TL, typename TB>
 __device__ __forceinline__ void line_transform(int kind, const TL &tl, int u, int TPL, int M, int N,
                                                unsigned long long plan, const TB &tw,
                                                const typename vec2_of<T>::type *twl, T *red)
 {
     switch (kind) {
-        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_1: line_transform_k<K_DST_1, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_2: line_transform_k<K_DST_2, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_3: line_transform_k<K_DST_3, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_R2C: line_transform_k<K_R2C, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        default: line_transform_k<K_C2R, EPT, T>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_1: line_transform_k<K_DST_1, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_2: line_transform_k<K_DST_2, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DST_3: line_transform_k<K_DST_3, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_R2C: line_transform_k<K_R2C, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        default: line_transform_k<K_C2R, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
     }
 }
 
