@@ -844,27 +844,36 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
     }
     __syncthreads();
     PassArgs q = p;
+    // the radix plan as a compile-time constant for the FFT stages (PC), except
+    // in fused z passes of 16 points per thread: unrolled there, the every-kind
+    // kernels kept their stage arrays in scratch (616 bytes per lane) and ran
+    // 2.5x slower, so those walk the plan at run time
+    constexpr unsigned long long PC = (OP == B_ZMID && EPT >= 16) ? 0ull : PLAN;
+    auto fwd = [&]() {
+        if constexpr (PC != 0) fft_forward_c<EPT, T, PC>(tl, u, TPL, M, twl);
+        else fft_forward<EPT, T>(tl, u, TPL, M, PLAN, twl);
+    };
     // the last transform of the pass; a C2R one leaves its unfold (a sign
     // flip of the odd slots) to the store
     int last = kind;
     if constexpr (OP == B_ZMID) {
         last = kind2;
         if (kind == K_R2C && kind2 == K_C2R) {
-            fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
+            fwd();
             if (sa.stencil19) periodic_split_divide<EPT, T, TL, true, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
             else periodic_split_divide<EPT, T, TL, false, 1>(tl, u, TPL, M, nlines, a_tile, b, sa, wkl, ezl);
-            fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
+            fwd();
         } else {
             auto zt = [&](const auto &tb) {
-                line_transform<EPT, T, PLAN>(kind, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                line_transform<EPT, T, PC>(kind, tl, u, TPL, M, N, PLAN, tb, twl, red);
                 if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 __syncthreads();
                 if (last == K_C2R) {
                     fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                    fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
+                    fwd();
                 } else {
-                    line_transform<EPT, T, PLAN>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                    line_transform<EPT, T, PC>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
                 }
             };
             if constexpr (FT) zt(ltab);
@@ -876,9 +885,9 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         auto xt = [&](const auto &tb) {
             if (last == K_C2R) {
                 fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                fft_forward_c<EPT, T, PLAN>(tl, u, TPL, M, twl);
+                fwd();
             } else {
-                line_transform<EPT, T, PLAN>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
+                line_transform<EPT, T, PC>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
             }
         };
         if constexpr (FT) xt(ltab);
