@@ -104,3 +104,27 @@ def test_emit_alt_and_write_table(tmp_path):
     assert G.read_table(str(alt)) == [(3, 64, 64, 8, 4, 16), (3, 64, 64, 8, 8, 8)]
     A.write_table(str(chosen), [(3, 64, 64, 8, 8, 8, 0.3, res), (1, 64, 64, 8, 4, 16, 0.2, res)], "# head\n")
     assert G.read_table(str(chosen)) == [(1, 64, 64, 8, 4, 16), (3, 64, 64, 8, 8, 8)]
+
+
+def test_zmid_role_specialisation(tmp_path):
+    # a role-5 row (fused z pass, non-periodic z) compiles no per-code kernels;
+    # the role-3 row of the same size is then specialised for code 0 only,
+    # without a role-5 row for every code
+    chosen = tmp_path / "c.txt"
+    alt = tmp_path / "a.txt"
+    alt.write_text("")
+    chosen.write_text("3 512 64 16 16 16\n5 512 64 8 32 16\n3 512 32 16 16 32\n")
+    ks = G.kernel_set([str(chosen), str(alt)])
+    spec = {(k[1], k[4], k[5], k[6]): set() for k in ks}
+    for k in ks:
+        if k[8] >= 0:
+            spec[(k[1], k[4], k[5], k[6])].add(k[8])
+    assert spec[(64, 16, 16, 16)] == {0}
+    assert spec[(64, 8, 32, 16)] == set()
+    assert spec[(32, 16, 16, 32)] == {0, 1, 2, 3, 4}
+    assert all(k[2] == G.B_ZMID for k in ks)
+
+
+def test_role5_rows_accepted():
+    assert G.check_row((5, 512, 64, 8, 32, 16)) == 0x40808
+    assert G.role_ops(G.ROLE_ZMIDN, 32) == [G.B_ZMID]
