@@ -869,12 +869,7 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
                 if (sa.stencil19) zdivide<T, true>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 else zdivide<T, false>(p, sa, tl, u, TPL, nlines, a_tile, b, ezl);
                 __syncthreads();
-                if (last == K_C2R) {
-                    fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                    fwd();
-                } else {
-                    line_transform<EPT, T, PC>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
-                }
+                line_transform<EPT, T, PC, true>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
             };
             if constexpr (FT) zt(ltab);
             else zt(tw);
@@ -882,14 +877,7 @@ __device__ __forceinline__ void run(const PassArgs &p, const SolveArgs &sa, cons
         q.out_lo = sa.out2_lo;
         q.out_hi = sa.out2_hi;
     } else {
-        auto xt = [&](const auto &tb) {
-            if (last == K_C2R) {
-                fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tb, red);
-                fwd();
-            } else {
-                line_transform<EPT, T, PC>(last, tl, u, TPL, M, N, PLAN, tb, twl, red);
-            }
-        };
+        auto xt = [&](const auto &tb) { line_transform<EPT, T, PC, true>(last, tl, u, TPL, M, N, PLAN, tb, twl, red); };
         if constexpr (FT) xt(ltab);
         else xt(tw);
     }

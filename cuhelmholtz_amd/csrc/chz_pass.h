@@ -489,36 +489,42 @@ template <typename T> struct Smem {
     }
 };
 
-// One line transform of the whole tile (fold, FFT, unfold), data in LDS.
-// PLANC != 0: the radix plan as a compile-time constant (fixed-shape kernels)
-template <int KIND, int EPT, typename T, unsigned long long PLANC = 0, typename TL, typename TB>
-__device__ __forceinline__ void line_transform_k(const TL &tl, int u, int TPL, int M, int N,
-                                                 unsigned long long plan, const TB &tw,
-                                                 const typename vec2_of<T>::type *twl, T *red)
-{
-    fold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
-    if constexpr (PLANC != 0) fft_forward_c<EPT, T, PLANC>(tl, u, TPL, M, twl);
-    else fft_forward<EPT, T>(tl, u, TPL, M, plan, twl);
-    unfold<KIND, EPT, T>(tl, u, TPL, M, N, tw, red);
-}
-
-// Runtime kind -> one specialised path (the branch is uniform over the workgroup).
-// tw: the pass's twiddle table in global memory, or the fold / unfold tables
-// staged in LDS (LTabs); twl: the FFT twiddles staged in LDS.
-template <int EPT, typename T, unsigned long long PLANC = 0, typename TL, typename TB>
+// One line transform of the whole tile, data in LDS: the runtime kind's fold,
+// the FFT, the kind's unfold (the branches are uniform over the workgroup).
+// The FFT, the bulk of the code, is emitted once for all the kinds rather than
+// once per kind.  PLANC != 0: the radix plan as a compile-time constant
+// (fixed-shape kernels).  C2R_AT_STORE: a C2R transform stops after the FFT,
+// its unfold (a sign flip of the odd slots) left to the store.  tw: the pass's
+// twiddle table in global memory, or the fold / unfold tables staged in LDS
+// (LTabs); twl: the FFT twiddles staged in LDS.
+template <int EPT, typename T, unsigned long long PLANC = 0, bool C2R_AT_STORE = false, typename TL, typename TB>
 __device__ __forceinline__ void line_transform(int kind, const TL &tl, int u, int TPL, int M, int N,
                                                unsigned long long plan, const TB &tw,
                                                const typename vec2_of<T>::type *twl, T *red)
 {
     switch (kind) {
-        case K_DCT_1: line_transform_k<K_DCT_1, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DCT_2: line_transform_k<K_DCT_2, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DCT_3: line_transform_k<K_DCT_3, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_1: line_transform_k<K_DST_1, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_2: line_transform_k<K_DST_2, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_DST_3: line_transform_k<K_DST_3, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        case K_R2C: line_transform_k<K_R2C, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
-        default: line_transform_k<K_C2R, EPT, T, PLANC>(tl, u, TPL, M, N, plan, tw, twl, red); break;
+        case K_DCT_1: fold<K_DCT_1, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DCT_2: fold<K_DCT_2, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DCT_3: fold<K_DCT_3, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_1: fold<K_DST_1, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_2: fold<K_DST_2, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_3: fold<K_DST_3, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_R2C: fold<K_R2C, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        default: fold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+    }
+    if constexpr (PLANC != 0) fft_forward_c<EPT, T, PLANC>(tl, u, TPL, M, twl);
+    else fft_forward<EPT, T>(tl, u, TPL, M, plan, twl);
+    switch (kind) {
+        case K_DCT_1: unfold<K_DCT_1, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DCT_2: unfold<K_DCT_2, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DCT_3: unfold<K_DCT_3, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_1: unfold<K_DST_1, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_2: unfold<K_DST_2, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_DST_3: unfold<K_DST_3, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        case K_R2C: unfold<K_R2C, EPT, T>(tl, u, TPL, M, N, tw, red); break;
+        default:
+            if constexpr (!C2R_AT_STORE) unfold<K_C2R, EPT, T>(tl, u, TPL, M, N, tw, red);
+            break;
     }
 }
 
