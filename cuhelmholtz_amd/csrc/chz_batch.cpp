@@ -9,8 +9,9 @@
 // workgroup per (problem, tile) item of its work list, reading every
 // problem's pass arguments from a device descriptor table.  The periodic
 // copies follow as three launches (x, y, z faces, in that order, as in the
-// single solve).  The whole sequence is captured into one hipGraph at plan
-// creation; a solve is one hipGraphLaunch.  The node arrays f of the problems
+// single solve).  The whole sequence is built into one hipGraph at plan
+// creation, the size classes of a step as parallel branches; a solve is one
+// hipGraphLaunch.  The node arrays f of the problems
 // are bound at creation: a solve reads the right-hand sides from them and
 // leaves the solutions there.
 #include <algorithm>
@@ -35,6 +36,7 @@ struct BatchLaunch {
     int nitems;
     const BatchDesc *descs;   // device
     std::string name;
+    int step;                 // pass index in the problems' sequences
 };
 
 // geometry of one problem of a plan (a rebind must present the same)
@@ -85,7 +87,6 @@ struct BatchPlan {
     std::vector<BatchLaunch> launches;
     const BatchCopy *copies[3] = {nullptr, nullptr, nullptr};
     int ncopies[3] = {0, 0, 0}, copy_grid[3] = {0, 0, 0};
-    hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // stream order across the streams a plan is used on: `done` marks the
@@ -100,7 +101,6 @@ struct BatchPlan {
         if (uploaded && up_rec) (void)hipEventSynchronize(uploaded);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (cap) (void)hipStreamDestroy(cap);
         if (tables) (void)hipFree(tables);
         if (work) (void)hipFree(work);
         if (staging) (void)hipHostFree(staging);
@@ -497,27 +497,58 @@ static int create(const chz_problem *probs, int count, int prec, int refine, Bat
         l.nitems = nitems[gi];
         l.descs = reinterpret_cast<const BatchDesc *>(tbase + gr.d_off);
         l.name = std::string(q.name) + "/" + std::to_string(q.N);
+        l.step = gr.step;
         if (l.nitems > 0) plan->launches.push_back(l);
     }
 
-    // capture the sequence into a graph
-    hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
-        delete plan;
-        return fail(ERR_DEVICE, std::string("graph capture start: ") + hipGetErrorString(e));
-    }
-    rc = enqueue(ctx, *plan, plan->cap, false);
+    // the graph: the launches of one step (one per size class, each over its
+    // own problems) are independent nodes that wait for every launch of the
+    // step before; the periodic copies follow, x, y, z faces in turn
     hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(plan->cap, &graph);
+    hipError_t e = hipGraphCreate(&graph, 0);
     plan->graph = graph;
-    if (rc) {
-        delete plan;
-        return rc;
+    std::vector<hipGraphNode_t> prev, cur;
+    int cur_step = -1;
+    for (const BatchLaunch &l : plan->launches) {
+        if (e != hipSuccess) break;
+        if (l.step != cur_step) {
+            if (!cur.empty()) prev.swap(cur);
+            cur.clear();
+            cur_step = l.step;
+        }
+        const BatchItem *items = l.items;
+        int n = l.nitems;
+        const BatchDesc *descs = l.descs;
+        void *args[] = {&items, &n, &descs};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(l.fn);
+        kp.gridDim = dim3((unsigned)l.grid);
+        kp.blockDim = dim3((unsigned)l.threads);
+        kp.sharedMemBytes = (unsigned)l.lds;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, graph, prev.data(), prev.size(), &kp);
+        if (e == hipSuccess) cur.push_back(node);
     }
-    if (e != hipSuccess || !graph) {
+    for (int a = 0; a < 3 && e == hipSuccess; a++) {
+        if (!plan->ncopies[a]) continue;
+        const BatchCopy *tab = plan->copies[a];
+        void *args[] = {&tab};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(batch_periodic_kernel());
+        kp.gridDim = dim3((unsigned)plan->copy_grid[a], (unsigned)plan->ncopies[a]);
+        kp.blockDim = dim3(256);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, graph, cur.data(), cur.size(), &kp);
+        if (e == hipSuccess) cur.assign(1, node);
+    }
+    if (e != hipSuccess) {
         delete plan;
-        return fail(ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        return fail(ERR_DEVICE, std::string("graph build: ") + hipGetErrorString(e));
     }
     if ((e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0)) != hipSuccess) {
         delete plan;
