@@ -110,6 +110,31 @@ def test_batch_graph_replays_and_eager_agree():
     bs.close()
 
 
+@pytest.mark.parametrize("prec,refine", [(64, 0), (32, -3)])
+def test_batch_graph_branches_replay_bitwise(prec, refine):
+    """The graph runs the size classes of a step as parallel branches: on the
+    benchmark's sizes (several classes per step; for FP32, per-problem
+    refinement counts, so the steps hold uneven problem sets) every one of
+    three replays, each after eager work and a synchronisation, equals the
+    same launches issued directly on one stream, bit for bit."""
+    probs = make(W.batched_problems(count=48, seed=5))
+    bs = C.BatchSolver([(p["f"], p["d"]["n"], p["d"]["bc"], p["d"]["lam"], p["bcs"]) for p in probs], prec, refine)
+    bs.solve(graph=False)
+    torch.cuda.synchronize()
+    want = [p["f"].clone() for p in probs]
+    for rep in range(3):
+        for p in probs:
+            p["f"].copy_(p["f0"])
+        torch.cuda.synchronize()
+        bs.solve(graph=True)
+        torch.cuda.synchronize()
+        for k, p in enumerate(probs):
+            assert torch.equal(p["f"], want[k]), (rep, k, p["d"])
+    for k, p in enumerate(probs):
+        assert rel(want[k], single(p)) <= (SAME if prec == 64 else 1e-10), (k, p["d"])
+    bs.close()
+
+
 def test_batch_benchmark_sizes_sample():
     """64 problems drawn like the 2048-problem workload (sizes 32..128)."""
     probs = make(W.batched_problems(count=64, seed=2))
