@@ -170,15 +170,16 @@ typedef struct chz_problem {
  * (chz_batch_refine_counts reports them).  The
  * plan owns one workspace per problem, device tables of per-problem pass
  * descriptors and its own copies of the twiddle and eigenvalue tables it
- * reads, and holds the whole solve captured as one hipGraph: one launch
- * per pass and size class, then the periodic copies.  The
+ * reads, and holds the whole solve as one hipGraph: one launch per pass
+ * and size class (the size classes of a pass as parallel branches), then
+ * the periodic copies.  The
  * problems' f arrays and face arrays are bound here: each chz_batch_solve
  * solves whatever they hold at that time.  Results equal the single-problem
  * solve's (same kernels and launch configurations).  No reference
  * counterpart (the reference solves one problem per call). */
 int chz_batch_create(const chz_problem *problems, int count, int precision, int refine, void **plan);
 
-/* Run the batched solve on `stream`: use_graph = 1 replays the captured
+/* Run the batched solve on `stream`: use_graph = 1 replays the plan's
  * graph (one hipGraphLaunch); 0 issues the same launches directly (and
  * records per-launch timing when chz_set_timing is on). */
 int chz_batch_solve(void *plan, int use_graph, void *stream);
